@@ -1,0 +1,664 @@
+// Counting kernels and the device half of the C ABI.
+//
+// One wave64 per workgroup; a persistent grid of waves pulls chunks of tasks
+// from a global atomic counter.  A task is a directed edge (v0, v1) = the
+// first two loop levels of the reference's nest (src/graph.cpp:213-225 and the
+// depth-1 loop of :373-396), or a root vertex when the pattern's leaf is at
+// depth 1.  From there the wave walks the remaining loop levels depth-first
+// with an explicit stack; every lane follows the same control path and the set
+// algebra of src/vertex_set.cpp is done cooperatively by the 64 lanes:
+//   - intersection: lanes take 64 elements of the shorter list at a time and
+//     binary-search the longer one, then ballot + popcount place the hits in
+//     order (a sorted result, like the reference's merge);
+//   - unorderd_subtraction_size: one lane per embedded vertex, binary search;
+//   - lower_bound for restriction cut-offs: 64-way search, about 3 probes
+//     for a 100k-element list.
+// Counts accumulate in unsigned 64-bit (two's-complement wrap-around, as the
+// reference's long long sums do in practice); the host divides by the IEP
+// redundancy once, after the global sum (src/graph.cpp:244).
+//
+// The FAITHFUL instantiation disables restriction truncation and leaf fusion
+// and replays the reference's intersection sequence (prefix build order with
+// its early exit, IEP group order with its zero early exit) so that it can
+// tally the algorithmic byte count B_alg: 4 bytes x (|A| + |B|) for every
+// intersection the reference performs plus 8 bytes per neighbour-list lookup.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../gp_internal.h"
+#include "kplan.h"
+#include "schedule.h"
+
+#define WAVE 64
+
+namespace {
+
+struct KArgs {
+    const KPlan* plan;
+    const uint32_t* vertex;
+    const int32_t* edge;
+    const int32_t* src;
+    int32_t v_cnt;
+    int32_t chunk;
+    uint64_t n_tasks;
+    unsigned long long* counter;
+    int32_t* scratch;
+    int64_t slot_stride;
+    int64_t wave_stride;
+    int64_t tmp_stride;
+    unsigned long long* out; // [0] raw sum, [1] B_alg bytes, [2] tasks taken
+};
+
+__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+
+__device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
+
+// sorted-list membership, one lane one probe sequence
+__device__ __forceinline__ bool contains(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+    int32_t lo = 0, len = n;
+    while (len > 0) {
+        int32_t half = len >> 1;
+        int32_t mid = lo + half;
+        bool right = a[mid] < x;
+        lo = right ? mid + 1 : lo;
+        len = right ? len - half - 1 : half;
+    }
+    return lo < n && a[lo] == x;
+}
+
+// first index with a[i] >= x; wave-uniform result
+__device__ int32_t wave_lower_bound(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+    const int lane = lane_id();
+    int32_t lo = 0, hi = n;
+    while (hi - lo > WAVE) {
+        int32_t step = (hi - lo + WAVE - 1) / WAVE;
+        int32_t pos = lo + lane * step;
+        bool lt = pos < hi && a[pos] < x;
+        int c = popc64(__ballot(lt));
+        if (c == 0) return lo;
+        int32_t nlo = lo + (c - 1) * step + 1;
+        int32_t nhi = min(hi, lo + c * step);
+        lo = nlo;
+        hi = nhi;
+    }
+    int32_t pos = lo + lane;
+    bool lt = pos < hi && a[pos] < x;
+    return lo + popc64(__ballot(lt));
+}
+
+// |{emb[j] : j < m} ∩ a[0:n)|
+__device__ int32_t count_emb_in(const int32_t* __restrict__ a, int32_t n, const int32_t* emb, int m) {
+    const int lane = lane_id();
+    bool hit = false;
+    if (lane < m && n > 0) hit = contains(a, n, emb[lane]);
+    return popc64(__ballot(hit));
+}
+
+// A ∩ B restricted to elements < ub.  COUNT: number of such elements not in
+// emb[0:m).  Otherwise the sorted result is written to out (no exclusion).
+template <bool COUNT>
+__device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const int32_t* __restrict__ B,
+                             int32_t nb, int32_t ub, int32_t* __restrict__ out, const int32_t* emb,
+                             int m) {
+    const int lane = lane_id();
+    const int32_t* S = A;
+    const int32_t* Lg = B;
+    int32_t ns = na, nl = nb;
+    if (na > nb) {
+        S = B;
+        Lg = A;
+        ns = nb;
+        nl = na;
+    }
+    const uint64_t below_me = (1ull << lane) - 1;
+    int32_t total = 0;
+    for (int32_t base = 0; base < ns; base += WAVE) {
+        int32_t i = base + lane;
+        bool in = i < ns;
+        int32_t x = in ? S[i] : 0;
+        bool keep = in && x < ub;
+        bool hit = keep && contains(Lg, nl, x);
+        if (COUNT)
+            for (int j = 0; j < m; ++j) hit = hit && x != emb[j];
+        uint64_t mk = __ballot(hit);
+        if (!COUNT && hit) out[total + popc64(mk & below_me)] = x;
+        total += popc64(mk);
+        if (__ballot(in && !keep)) break; // the rest of S is >= ub
+    }
+    return total;
+}
+
+template <bool FAITHFUL>
+__global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
+    __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
+    __shared__ int32_t s_len[GP_MAX_PREFIX];
+    __shared__ int32_t s_emb[GP_MAX_SIZE];
+    __shared__ const int32_t* s_lp[GP_MAX_SIZE];
+    __shared__ int32_t s_ll[GP_MAX_SIZE];
+    __shared__ int32_t s_li[GP_MAX_SIZE];
+    __shared__ int32_t s_blk[GP_MAX_BLOCKS];
+
+    const KPlan& P = *a.plan;
+    const int lane = lane_id();
+    const int n = P.n, L = P.leaf, K = P.k;
+    const int32_t v_cnt = a.v_cnt;
+    int32_t* wbase = a.scratch + (int64_t)blockIdx.x * a.wave_stride;
+    uint64_t acc = 0, balg = 0, taken = 0;
+
+    auto bound = [&](uint32_t mask) -> int32_t {
+        int32_t b = v_cnt;
+        for (int j = 0; j < n; ++j)
+            if (mask >> j & 1) b = min(b, s_emb[j]);
+        return b;
+    };
+    auto ub_of = [&](int p) -> int32_t {
+        if (FAITHFUL || P.ub_n[p] == 0) return INT_MAX;
+        int32_t ub = 0;
+        for (int i = 0; i < P.ub_n[p]; ++i) ub = max(ub, bound(P.ub_mask[p][i]));
+        return ub;
+    };
+    auto build_prefix = [&](int p, const int32_t* nv, int32_t deg) -> int32_t {
+        const int f = P.father[p];
+        const int32_t ub = ub_of(p);
+        const int32_t* ptr;
+        int32_t len;
+        if (f < 0) {
+            ptr = nv;
+            len = ub == INT_MAX ? deg : wave_lower_bound(nv, deg, ub);
+        } else {
+            int32_t* out = wbase + (int64_t)P.slot[p] * a.slot_stride;
+            const int32_t* fp = s_ptr[f];
+            const int32_t fl = s_len[f];
+            if (FAITHFUL) balg += 4ull * (uint64_t)((int64_t)fl + deg);
+            len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0);
+            ptr = out;
+            __syncthreads(); // make the new set visible to every lane
+        }
+        s_ptr[p] = ptr;
+        s_len[p] = len;
+        return len;
+    };
+    // build the prefixes whose last vertex is depth d; false when one is empty
+    auto build_level = [&](int d, const int32_t* nv, int32_t deg) -> bool {
+        for (int i = 0; i < P.nbuild[d]; ++i) {
+            const int p = P.build[d][i];
+            if (!FAITHFUL && p == P.fused_prefix) continue; // counted by the leaf
+            if (build_prefix(p, nv, deg) == 0) return false;
+        }
+        return true;
+    };
+    // count of the common elements of the block's sets, minus embedded ones
+    auto block_count = [&](uint32_t bm) -> int32_t {
+        int best = -1;
+        int32_t best_len = INT_MAX;
+        for (int i = 0; i < K; ++i)
+            if (bm >> i & 1) {
+                int q = P.loop_set[L + i];
+                if (s_len[q] < best_len) {
+                    best_len = s_len[q];
+                    best = q;
+                }
+            }
+        const int32_t* S = s_ptr[best];
+        int32_t total = 0;
+        for (int32_t base = 0; base < best_len; base += WAVE) {
+            int32_t i = base + lane;
+            bool hit = i < best_len;
+            int32_t x = hit ? S[i] : 0;
+            for (int t = 0; t < K; ++t) {
+                if (!(bm >> t & 1)) continue;
+                int q = P.loop_set[L + t];
+                if (q == best) continue;
+                hit = hit && contains(s_ptr[q], s_len[q], x);
+            }
+            for (int j = 0; j < L; ++j) hit = hit && x != s_emb[j];
+            total += popc64(__ballot(hit));
+        }
+        return total;
+    };
+    // count at depth L with emb[0:L) placed; nv/deg = N(emb[L-1])
+    auto leaf = [&](const int32_t* nv, int32_t deg) -> uint64_t {
+        if (K == 0) {
+            const int q = P.loop_set[L];
+            const int32_t b = bound(P.rmask[L]);
+            if (!FAITHFUL && P.fused) {
+                const int f = P.father[q];
+                return (uint64_t)(int64_t)intersect<true>(s_ptr[f], s_len[f], nv, deg, b, nullptr, s_emb, L);
+            }
+            const int32_t* S = s_ptr[q];
+            int32_t c = s_len[q];
+            if (P.rmask[L]) c = wave_lower_bound(S, c, b);
+            if (c > 0) c -= count_emb_in(S, c, s_emb, L);
+            return (uint64_t)(int64_t)c;
+        }
+        if (!FAITHFUL) {
+            for (int bi = 0; bi < P.nblocks; ++bi) {
+                const uint32_t bm = P.block_mask[bi];
+                int32_t c;
+                if (__popc(bm) == 1) {
+                    const int q = P.loop_set[L + __ffs(bm) - 1];
+                    c = s_len[q] - count_emb_in(s_ptr[q], s_len[q], s_emb, L);
+                } else {
+                    c = block_count(bm);
+                }
+                s_blk[bi] = c;
+            }
+            uint64_t sum = 0;
+            for (int g = 0; g < P.ngroups; ++g) {
+                uint64_t val = (uint64_t)P.group_val[g];
+                for (int j = 0; j < P.group_nb[g]; ++j) val *= (uint64_t)(int64_t)s_blk[P.group_blk[g][j]];
+                sum += val;
+            }
+            return sum;
+        }
+        // reference order: group by group, block by block, stop at zero
+        uint64_t sum = 0;
+        int32_t* tmp0 = wbase + (int64_t)P.nslots * a.slot_stride;
+        int32_t* tmp1 = tmp0 + a.tmp_stride;
+        if (s_len[P.loop_set[L]] <= 0) return 0; // src/graph.cpp:250-252
+        for (int g = 0; g < P.ngroups; ++g) {
+            uint64_t val = (uint64_t)P.group_val[g];
+            for (int j = 0; j < P.group_nb[g]; ++j) {
+                const uint32_t bm = P.block_mask[P.group_blk[g][j]];
+                int32_t c;
+                if (__popc(bm) == 1) {
+                    const int q = P.loop_set[L + __ffs(bm) - 1];
+                    c = s_len[q] - count_emb_in(s_ptr[q], s_len[q], s_emb, L);
+                } else {
+                    int first = -1;
+                    const int32_t* cur = nullptr;
+                    int32_t cn = 0;
+                    int32_t* dst = tmp0;
+                    for (int t = 0; t < K; ++t) {
+                        if (!(bm >> t & 1)) continue;
+                        const int q = P.loop_set[L + t];
+                        if (first < 0) {
+                            first = q;
+                            continue;
+                        }
+                        const int32_t* A = cur ? cur : s_ptr[first];
+                        const int32_t an = cur ? cn : s_len[first];
+                        balg += 4ull * (uint64_t)((int64_t)an + s_len[q]);
+                        cn = intersect<false>(A, an, s_ptr[q], s_len[q], INT_MAX, dst, nullptr, 0);
+                        __syncthreads();
+                        cur = dst;
+                        dst = dst == tmp0 ? tmp1 : tmp0;
+                    }
+                    c = cn - (cn > 0 ? count_emb_in(cur, cn, s_emb, L) : 0);
+                }
+                val *= (uint64_t)(int64_t)c;
+                if (val == 0) break;
+            }
+            sum += val;
+        }
+        return sum;
+    };
+    auto enter = [&](int d) {
+        const int q = P.loop_set[d];
+        s_lp[d] = s_ptr[q];
+        int32_t len = s_len[q];
+        if (P.rmask[d]) len = wave_lower_bound(s_ptr[q], len, bound(P.rmask[d]));
+        s_ll[d] = len;
+        s_li[d] = 0;
+    };
+
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
+        base = __shfl(base, 0);
+        if (base >= a.n_tasks) break;
+        const uint64_t end = min((uint64_t)base + (uint64_t)a.chunk, a.n_tasks);
+        for (uint64_t t = base; t < end; ++t) {
+            ++taken;
+            int32_t v0, v1 = -1;
+            if (P.root_mode) {
+                v0 = (int32_t)t;
+            } else {
+                v0 = a.src[t];
+                v1 = a.edge[t];
+                if (v1 == v0) continue;
+                if ((P.rmask[1] & 1) && v1 >= v0) continue;
+            }
+            s_emb[0] = v0;
+            const uint32_t l0 = a.vertex[v0];
+            const int32_t* n0 = a.edge + l0;
+            const int32_t d0 = (int32_t)(a.vertex[v0 + 1] - l0);
+            for (int i = 0; i < P.nbuild[0]; ++i) build_prefix(P.build[0][i], n0, d0);
+            if (P.root_mode) {
+                acc += leaf(n0, d0);
+                continue;
+            }
+            s_emb[1] = v1;
+            if (FAITHFUL) balg += 8;
+            const uint32_t l1 = a.vertex[v1];
+            const int32_t* n1 = a.edge + l1;
+            const int32_t d1 = (int32_t)(a.vertex[v1 + 1] - l1);
+            if (!build_level(1, n1, d1)) continue;
+            if (L == 2) {
+                acc += leaf(n1, d1);
+                continue;
+            }
+            // depth-first walk over depths 2 .. L-1
+            int d = 2;
+            enter(d);
+            for (;;) {
+                const int32_t idx = s_li[d];
+                if (idx >= s_ll[d]) {
+                    if (d == 2) break;
+                    --d;
+                    continue;
+                }
+                s_li[d] = idx + 1;
+                const int32_t v = s_lp[d][idx];
+                bool dup = false;
+                for (int j = 0; j < d; ++j) dup = dup || s_emb[j] == v;
+                if (dup) continue;
+                s_emb[d] = v;
+                if (FAITHFUL) balg += 8;
+                const uint32_t lv = a.vertex[v];
+                const int32_t* nv = a.edge + lv;
+                const int32_t dv = (int32_t)(a.vertex[v + 1] - lv);
+                if (!build_level(d, nv, dv)) continue;
+                if (d == L - 1) {
+                    acc += leaf(nv, dv);
+                    continue;
+                }
+                ++d;
+                enter(d);
+            }
+        }
+    }
+    if (lane == 0) {
+        atomicAdd(&a.out[0], (unsigned long long)acc);
+        if (FAITHFUL) atomicAdd(&a.out[1], (unsigned long long)balg);
+        atomicAdd(&a.out[2], (unsigned long long)taken);
+    }
+}
+
+// src[e] = the vertex whose list holds edge slot e
+__global__ void build_src_kernel(const uint32_t* __restrict__ vertex, int32_t v_cnt, uint32_t e_cnt,
+                                 int32_t* __restrict__ src) {
+    uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e_cnt) return;
+    int32_t lo = 0, hi = v_cnt; // last v with vertex[v] <= e
+    while (hi - lo > 1) {
+        int32_t mid = lo + (hi - lo) / 2;
+        if (vertex[mid] <= (uint32_t)e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    src[e] = lo;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, int n) {
+    int i = (int)threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// runtime
+// ---------------------------------------------------------------------------
+
+struct CallCtx {
+    KPlan* d_plan = nullptr;
+    unsigned long long* d_cnt = nullptr; // [0] task counter, [1..3] outputs
+    int32_t* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+struct gp_graph {
+    int device = 0;
+    int num_cu = 0;
+    int32_t v_cnt = 0;
+    uint32_t e_cnt = 0;
+    int32_t max_deg = 0, max_deg2 = 0;
+    uint32_t* d_vertex = nullptr;
+    int32_t* d_edge = nullptr;
+    int32_t* d_src = nullptr;
+    std::mutex mu;
+    std::vector<CallCtx*> idle;
+};
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t err_ = (expr);                                                                \
+        if (err_ != hipSuccess)                                                                  \
+            return gp_fail((int)err_, std::string(#expr) + ": " + hipGetErrorString(err_));      \
+    } while (0)
+
+namespace {
+
+void destroy_ctx(CallCtx* c) {
+    if (!c) return;
+    if (c->d_plan) (void)hipFree(c->d_plan);
+    if (c->d_cnt) (void)hipFree(c->d_cnt);
+    if (c->d_scratch) (void)hipFree(c->d_scratch);
+    if (c->e0) (void)hipEventDestroy(c->e0);
+    if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int acquire_ctx(gp_graph* g, size_t scratch_bytes, CallCtx** out) {
+    CallCtx* c = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (!g->idle.empty()) {
+            c = g->idle.back();
+            g->idle.pop_back();
+        }
+    }
+    if (!c) {
+        c = new CallCtx();
+        hipError_t e = hipSuccess;
+        if (e == hipSuccess) e = hipMalloc((void**)&c->d_plan, sizeof(KPlan));
+        if (e == hipSuccess) e = hipMalloc((void**)&c->d_cnt, 4 * sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&c->e0);
+        if (e == hipSuccess) e = hipEventCreate(&c->e1);
+        if (e != hipSuccess) {
+            destroy_ctx(c);
+            return gp_fail((int)e, std::string("call context: ") + hipGetErrorString(e));
+        }
+    }
+    if (c->scratch_bytes < scratch_bytes) {
+        if (c->d_scratch) (void)hipFree(c->d_scratch);
+        c->d_scratch = nullptr;
+        c->scratch_bytes = 0;
+        hipError_t e = hipMalloc((void**)&c->d_scratch, scratch_bytes);
+        if (e != hipSuccess) {
+            destroy_ctx(c);
+            return gp_fail(GP_ENOMEM, std::string("scratch allocation of ") + std::to_string(scratch_bytes) +
+                                          " bytes failed: " + hipGetErrorString(e));
+        }
+        c->scratch_bytes = scratch_bytes;
+    }
+    *out = c;
+    return GP_OK;
+}
+
+void release_ctx(gp_graph* g, CallCtx* c) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->idle.push_back(c);
+}
+
+int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw,
+              int64_t* count, gp_stats* stats) {
+    const bool faithful = tuning && tuning->collect_balg;
+    KPlan kp;
+    int rc = gp_make_kplan(sched, faithful, &kp);
+    if (rc != GP_OK) return rc;
+    if (tuning && tuning->no_truncate) {
+        std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
+    }
+    HIP_TRY(hipSetDevice(g->device));
+    int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 16;
+    int chunk = tuning && tuning->chunk > 0 ? tuning->chunk : 16;
+    int64_t waves = (int64_t)g->num_cu * wpc;
+    const uint64_t n_tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+    if (waves < 1) waves = 1;
+    if ((uint64_t)waves > n_tasks / (uint64_t)chunk + 1) waves = (int64_t)(n_tasks / (uint64_t)chunk + 1);
+    // scratch: per wave, one slot per materialised prefix (a subset of one
+    // neighbour list of a vertex other than the root, so at most the
+    // second-largest degree), plus two IEP temporaries in faithful mode
+    const int64_t slot_stride = ((int64_t)std::max(g->max_deg2, 1) + 63) / 64 * 64;
+    const int64_t tmp_stride = faithful && kp.k ? ((int64_t)std::max(g->max_deg, 1) + 63) / 64 * 64 : 0;
+    const int64_t wave_stride = kp.nslots * slot_stride + 2 * tmp_stride;
+    const size_t scratch_bytes = (size_t)std::max<int64_t>(waves * wave_stride, 64) * sizeof(int32_t);
+
+    CallCtx* c = nullptr;
+    rc = acquire_ctx(g, scratch_bytes, &c);
+    if (rc != GP_OK) return rc;
+    KArgs a;
+    a.plan = c->d_plan;
+    a.vertex = g->d_vertex;
+    a.edge = g->d_edge;
+    a.src = g->d_src;
+    a.v_cnt = g->v_cnt;
+    a.chunk = chunk;
+    a.n_tasks = n_tasks;
+    a.counter = c->d_cnt;
+    a.scratch = c->d_scratch;
+    a.slot_stride = slot_stride;
+    a.wave_stride = wave_stride;
+    a.tmp_stride = tmp_stride;
+    a.out = c->d_cnt + 1;
+    unsigned long long host_out[4] = {0, 0, 0, 0};
+    hipError_t e = hipMemcpyAsync(c->d_plan, &kp, sizeof(KPlan), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt, 4);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+    if (e == hipSuccess && n_tasks > 0) {
+        if (faithful)
+            hipLaunchKernelGGL(gp_count_kernel<true>, dim3((unsigned)waves), dim3(WAVE), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL(gp_count_kernel<false>, dim3((unsigned)waves), dim3(WAVE), 0, c->stream, a);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
+    if (e != hipSuccess) {
+        // the context may hold a poisoned stream; drop it
+        destroy_ctx(c);
+        return gp_fail((int)e, std::string("count kernel: ") + hipGetErrorString(e));
+    }
+    release_ctx(g, c);
+    const uint64_t sum = host_out[1];
+    if (raw) *raw = sum;
+    if (count) *count = (int64_t)sum / (sched->redundancy ? sched->redundancy : 1);
+    if (stats) {
+        stats->kernel_ms = ms;
+        // every root's neighbour list is looked up once by the reference
+        stats->balg_bytes = faithful ? host_out[2] + 8ull * (uint64_t)g->v_cnt : 0;
+        stats->tasks = host_out[3];
+        stats->waves = (int32_t)waves;
+        stats->kernel_variant = faithful ? 1 : 0;
+    }
+    return GP_OK;
+}
+
+} // namespace
+
+extern "C" int gp_device_count(int32_t* n) {
+    if (!n) return gp_fail(GP_EINVAL, "null argument");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *n = e == hipSuccess ? c : 0;
+    return GP_OK;
+}
+
+extern "C" int gp_graph_upload(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt, uint32_t e_cnt,
+                               gp_graph** out) {
+    if (!out || !vertex || (!edge && e_cnt) || v_cnt < 0) return gp_fail(GP_EINVAL, "bad CSR arguments");
+    if (vertex[v_cnt] != e_cnt) return gp_fail(GP_EINVAL, "vertex[v_cnt] != e_cnt");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return gp_fail(GP_ENODEV, "no HIP device");
+    gp_graph* g = new gp_graph();
+    HIP_TRY(hipGetDevice(&g->device));
+    HIP_TRY(hipDeviceGetAttribute(&g->num_cu, hipDeviceAttributeMultiprocessorCount, g->device));
+    g->v_cnt = v_cnt;
+    g->e_cnt = e_cnt;
+    for (int32_t v = 0; v < v_cnt; ++v) {
+        int32_t d = (int32_t)(vertex[v + 1] - vertex[v]);
+        if (d > g->max_deg) {
+            g->max_deg2 = g->max_deg;
+            g->max_deg = d;
+        } else if (d > g->max_deg2) {
+            g->max_deg2 = d;
+        }
+    }
+    auto fail = [&](hipError_t e, const char* what) {
+        gp_graph_free(g);
+        return gp_fail((int)e, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void**)&g->d_vertex, sizeof(uint32_t) * ((size_t)v_cnt + 1))) != hipSuccess) return fail(e, "hipMalloc vertex");
+    if ((e = hipMalloc((void**)&g->d_edge, sizeof(int32_t) * std::max<size_t>(e_cnt, 1))) != hipSuccess) return fail(e, "hipMalloc edge");
+    if ((e = hipMalloc((void**)&g->d_src, sizeof(int32_t) * std::max<size_t>(e_cnt, 1))) != hipSuccess) return fail(e, "hipMalloc src");
+    if ((e = hipMemcpy(g->d_vertex, vertex, sizeof(uint32_t) * ((size_t)v_cnt + 1), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy vertex");
+    if (e_cnt && (e = hipMemcpy(g->d_edge, edge, sizeof(int32_t) * (size_t)e_cnt, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "copy edge");
+    if (e_cnt) {
+        unsigned blocks = (unsigned)(((uint64_t)e_cnt + 255) / 256);
+        hipLaunchKernelGGL(build_src_kernel, dim3(blocks), dim3(256), 0, 0, g->d_vertex, v_cnt, e_cnt, g->d_src);
+        if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build_src_kernel");
+        if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "build_src_kernel sync");
+    }
+    *out = g;
+    return GP_OK;
+}
+
+extern "C" int gp_graph_free(gp_graph* g) {
+    if (!g) return GP_OK;
+    (void)hipSetDevice(g->device);
+    for (CallCtx* c : g->idle) destroy_ctx(c);
+    if (g->d_vertex) (void)hipFree(g->d_vertex);
+    if (g->d_edge) (void)hipFree(g->d_edge);
+    if (g->d_src) (void)hipFree(g->d_src);
+    delete g;
+    return GP_OK;
+}
+
+extern "C" int gp_graph_bytes(const gp_graph* g, uint64_t* bytes) {
+    if (!g || !bytes) return gp_fail(GP_EINVAL, "null argument");
+    *bytes = sizeof(uint32_t) * ((uint64_t)g->v_cnt + 1) + 2 * sizeof(int32_t) * (uint64_t)g->e_cnt;
+    return GP_OK;
+}
+
+extern "C" int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw_sum,
+                        int64_t* count, gp_stats* stats) {
+    if (!g || !sched) return gp_fail(GP_EINVAL, "null argument");
+    return run_count(const_cast<gp_graph*>(g), sched, tuning, raw_sum, count, stats);
+}
+
+extern "C" int gp_tc(const gp_graph* g, uint64_t* triangles, gp_stats* stats) {
+    if (!g || !triangles) return gp_fail(GP_EINVAL, "null argument");
+    // triangle with v1 < v0 and v2 < v1: each triangle once
+    const int adj[9] = {0, 1, 1, 1, 0, 1, 1, 1, 0};
+    Schedule s(adj, 3);
+    s.add_restrict({{0, 1}, {1, 2}});
+    gp_sched_flat flat;
+    int rc = s.flatten(&flat);
+    if (rc != GP_OK) return gp_fail(rc, "triangle schedule");
+    uint64_t raw = 0;
+    rc = run_count(const_cast<gp_graph*>(g), &flat, nullptr, &raw, nullptr, stats);
+    if (rc != GP_OK) return rc;
+    *triangles = raw;
+    return GP_OK;
+}

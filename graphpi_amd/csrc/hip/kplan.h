@@ -1,0 +1,38 @@
+// Kernel plan: the flattened schedule recompiled into the small tables the
+// counting kernel walks.  Built on the host per gp_count call.
+#pragma once
+#include <cstdint>
+
+#include "graphpi_amd.h"
+
+#define GP_MAX_UB_MASKS 4
+#define GP_MAX_BLOCKS 63
+
+struct KPlan {
+    int32_t n;           // pattern size
+    int32_t leaf;        // depth whose count is evaluated without a loop
+    int32_t k;           // IEP width (0 = plain leaf)
+    int32_t nprefix;
+    int32_t nslots;      // materialised prefix sets per wave
+    int32_t fused;       // plain leaf counted while building its set at leaf-1
+    int32_t fused_prefix;
+    int32_t root_mode;   // leaf == 1: one task per root vertex
+    int8_t father[GP_MAX_PREFIX];
+    int8_t slot[GP_MAX_PREFIX];      // scratch slot, -1 when not materialised
+    int8_t depth_of[GP_MAX_PREFIX];  // depth at which the prefix is built
+    uint8_t ub_n[GP_MAX_PREFIX];     // 0 = keep every element
+    uint8_t ub_mask[GP_MAX_PREFIX][GP_MAX_UB_MASKS];
+    uint8_t rmask[GP_MAX_SIZE];      // restriction bound at each depth: min emb[j], j in mask
+    int8_t loop_set[GP_MAX_SIZE];
+    int8_t nbuild[GP_MAX_SIZE];
+    int8_t build[GP_MAX_SIZE][GP_MAX_PREFIX];
+    int32_t nblocks;                  // distinct IEP blocks (subsets of the k trailing vertices)
+    uint8_t block_mask[GP_MAX_BLOCKS];
+    int32_t ngroups;
+    uint8_t group_nb[GP_MAX_IEP_GROUPS];
+    uint8_t group_blk[GP_MAX_IEP_GROUPS][GP_MAX_IEP_K];
+    int64_t group_val[GP_MAX_IEP_GROUPS];
+};
+
+// Returns GP_OK or an error code (with gp_fail message) for unsupported input.
+int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* out);

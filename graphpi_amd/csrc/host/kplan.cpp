@@ -1,0 +1,189 @@
+// Builds the kernel plan from a flattened schedule.
+//
+// Truncation by restriction bounds.  A prefix set built at depth dp is read by
+//   - the loop at depth d, which stops at bound(d) = min emb[j] over the
+//     restriction indices j keyed at d (src/graph.cpp:367-376),
+//   - the plain leaf, which counts elements below bound(n-1) (:350-359),
+//   - the IEP leaf, which reads every element (:306-344),
+//   - its child prefixes (father of), through their own readers.
+// So only elements below max over readers of (that reader's bound) are ever
+// used, and the set may be cut there when it is built.  Bounds that mention a
+// vertex not yet placed at depth dp (index j > dp) are replaced by that
+// vertex's own bound: emb[j] < min emb[i] over i in rmask[j], because emb[j]
+// passed the loop bound at depth j.  A reader with no bound, or a bound that
+// cannot be expressed with placed vertices, keeps the whole set.  Every count
+// stays identical: each reader still sees all elements below its own bound.
+#include "../hip/kplan.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../gp_internal.h"
+
+namespace {
+
+// rewrite a bound mask so it only mentions vertices placed at depth <= dp;
+// returns 0 when that is impossible (unbounded)
+uint32_t restrict_mask(uint32_t m, int dp, const uint8_t* rmask) {
+    for (int guard = 0; guard < 64; ++guard) {
+        uint32_t high = m & ~((2u << dp) - 1);
+        if (!high) return m;
+        int j = 31 - __builtin_clz(high);
+        if (rmask[j] == 0) return 0;
+        m = (m & ~(1u << j)) | rmask[j];
+    }
+    return 0;
+}
+
+} // namespace
+
+int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* kp) {
+    std::memset(kp, 0, sizeof(*kp));
+    const int n = s->size;
+    if (n < 2 || n > GP_MAX_SIZE) return gp_fail(GP_ERANGE, "pattern size must be 2..8");
+    const int P = s->total_prefix_num;
+    if (P < 0 || P > GP_MAX_PREFIX) return gp_fail(GP_ERANGE, "too many prefixes");
+    const int k = s->iep_num > 1 ? s->iep_num : 0;
+    if (k > GP_MAX_IEP_K || k >= n) return gp_fail(GP_ERANGE, "IEP width out of range");
+    if (s->redundancy == 0) return gp_fail(GP_EINVAL, "redundancy is zero");
+    kp->n = n;
+    kp->k = k;
+    kp->leaf = k ? n - k : n - 1;
+    kp->nprefix = P;
+    kp->root_mode = kp->leaf == 1;
+    for (int p = 0; p < GP_MAX_PREFIX; ++p) {
+        kp->father[p] = -1;
+        kp->slot[p] = -1;
+        kp->depth_of[p] = -1;
+    }
+    for (int p = 0; p < P; ++p) {
+        if (s->father[p] < -1 || s->father[p] >= P) return gp_fail(GP_EINVAL, "bad father prefix id");
+        kp->father[p] = (int8_t)s->father[p];
+    }
+    for (int d = 0; d < n; ++d) {
+        kp->loop_set[d] = (int8_t)s->loop_set[d];
+        if (d >= 1 && (s->loop_set[d] < 0 || s->loop_set[d] >= P)) return gp_fail(GP_EINVAL, "bad loop set id");
+        int cnt = 0;
+        for (int p = s->last[d]; p != -1; p = s->next[p]) {
+            if (p < 0 || p >= P || cnt >= GP_MAX_PREFIX) return gp_fail(GP_EINVAL, "bad prefix list");
+            kp->build[d][cnt++] = (int8_t)p;
+            kp->depth_of[p] = (int8_t)d;
+        }
+        kp->nbuild[d] = (int8_t)cnt;
+        uint32_t m = 0;
+        int guard = 0;
+        for (int r = s->restrict_last[d]; r != -1; r = s->restrict_next[r]) {
+            if (r < 0 || r >= GP_MAX_RESTRICT || ++guard > GP_MAX_RESTRICT) return gp_fail(GP_EINVAL, "bad restriction list");
+            int idx = s->restrict_index[r];
+            if (idx < 0 || idx >= d) return gp_fail(GP_EINVAL, "restriction index must precede its depth");
+            m |= 1u << idx;
+        }
+        kp->rmask[d] = (uint8_t)m;
+    }
+    for (int p = 0; p < P; ++p)
+        if (kp->depth_of[p] < 0 || kp->depth_of[p] >= kp->leaf)
+            return gp_fail(GP_EINVAL, "prefix built outside the loop depths");
+
+    // ---- truncation bounds ----
+    std::vector<std::vector<uint32_t> > readers(P);
+    std::vector<char> unbounded(P, faithful ? 1 : 0);
+    auto add_reader = [&](int p, uint32_t mask) {
+        if (mask == 0)
+            unbounded[p] = 1;
+        else
+            readers[p].push_back(mask);
+    };
+    if (!faithful) {
+        for (int d = 1; d < kp->leaf; ++d) add_reader(kp->loop_set[d], kp->rmask[d]);
+        if (k == 0)
+            add_reader(kp->loop_set[kp->leaf], kp->rmask[kp->leaf]);
+        else
+            for (int i = 0; i < k; ++i) unbounded[kp->loop_set[kp->leaf + i]] = 1;
+        // children before fathers: deeper prefixes first
+        std::vector<int> order(P);
+        for (int p = 0; p < P; ++p) order[p] = p;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return kp->depth_of[a] > kp->depth_of[b]; });
+        for (int c : order) {
+            // rewrite c's masks for its own depth, then hand them to its father
+            std::vector<uint32_t> mine;
+            for (uint32_t m : readers[c]) {
+                uint32_t r = restrict_mask(m, kp->depth_of[c], kp->rmask);
+                if (!r) {
+                    unbounded[c] = 1;
+                    break;
+                }
+                mine.push_back(r);
+            }
+            if (unbounded[c]) mine.clear();
+            // drop masks whose bound can never exceed another's (supersets)
+            std::vector<uint32_t> minimal;
+            for (uint32_t m : mine) {
+                bool dominated = false;
+                for (uint32_t o : mine)
+                    if (o != m && (o & m) == o) dominated = true;
+                if (!dominated && std::find(minimal.begin(), minimal.end(), m) == minimal.end()) minimal.push_back(m);
+            }
+            if (minimal.size() > GP_MAX_UB_MASKS) unbounded[c] = 1;
+            readers[c] = minimal;
+            int f = kp->father[c];
+            if (f >= 0) {
+                if (unbounded[c])
+                    unbounded[f] = 1;
+                else
+                    for (uint32_t m : readers[c]) readers[f].push_back(m);
+            }
+        }
+        for (int p = 0; p < P; ++p) {
+            if (unbounded[p] || readers[p].empty()) continue;
+            kp->ub_n[p] = (uint8_t)readers[p].size();
+            for (size_t i = 0; i < readers[p].size(); ++i) kp->ub_mask[p][i] = (uint8_t)readers[p][i];
+        }
+    }
+
+    // ---- fused plain leaf ----
+    kp->fused = 0;
+    kp->fused_prefix = -1;
+    if (!faithful && k == 0 && kp->leaf >= 2) {
+        int q = kp->loop_set[kp->leaf];
+        if (kp->depth_of[q] == kp->leaf - 1 && kp->nbuild[kp->leaf - 1] == 1 && kp->father[q] >= 0) {
+            kp->fused = 1;
+            kp->fused_prefix = q;
+        }
+    }
+
+    // ---- scratch slots ----
+    int slots = 0;
+    for (int p = 0; p < P; ++p)
+        if (kp->father[p] >= 0 && p != kp->fused_prefix) kp->slot[p] = (int8_t)slots++;
+    kp->nslots = slots;
+
+    // ---- IEP tables ----
+    if (k) {
+        if (s->iep_group_num < 1 || s->iep_group_num > GP_MAX_IEP_GROUPS) return gp_fail(GP_EINVAL, "bad IEP group count");
+        kp->ngroups = s->iep_group_num;
+        for (int g = 0; g < s->iep_group_num; ++g) {
+            int nb = 0;
+            for (int i = 0; i < k; ++i) nb = std::max(nb, (int)s->iep_block[g][i] + 1);
+            if (nb < 1 || nb > k) return gp_fail(GP_EINVAL, "bad IEP partition");
+            kp->group_nb[g] = (uint8_t)nb;
+            for (int b = 0; b < nb; ++b) {
+                uint32_t m = 0;
+                for (int i = 0; i < k; ++i)
+                    if (s->iep_block[g][i] == b) m |= 1u << i;
+                if (!m) return gp_fail(GP_EINVAL, "empty IEP block");
+                int idx = -1;
+                for (int j = 0; j < kp->nblocks; ++j)
+                    if (kp->block_mask[j] == m) idx = j;
+                if (idx < 0) {
+                    if (kp->nblocks >= GP_MAX_BLOCKS) return gp_fail(GP_ERANGE, "too many IEP blocks");
+                    idx = kp->nblocks++;
+                    kp->block_mask[idx] = (uint8_t)m;
+                }
+                kp->group_blk[g][b] = (uint8_t)idx;
+            }
+            kp->group_val[g] = s->iep_val[g];
+        }
+    }
+    return GP_OK;
+}

@@ -1,0 +1,192 @@
+/*
+ * graphpi_amd.h — C ABI of the MI355X counting engine.
+ *
+ * Plain C types only (no C++ or torch types), so cgo / JNI / ctypes / N-API
+ * bindings can declare these directly.  Every function returns an int status:
+ * GP_OK (0) on success, a positive hipError_t value for HIP failures, or a
+ * negative GP_E* code for API misuse.  gp_last_error() returns the text of the
+ * calling thread's most recent failure.  No exception crosses this boundary.
+ *
+ * Reference interfaces replaced (thu-pacman/GraphPi, paths relative to the
+ * reference repository):
+ *   - Graph::pattern_matching(const Schedule&, int, bool)   include/graph.h:39,
+ *     src/graph.cpp:199-398 (the OpenMP loop nest)          -> gp_count
+ *   - VertexSet set algebra  src/vertex_set.cpp:56-209       -> kernels behind gp_count
+ *   - Graph::triangle_counting_mt  src/graph.cpp:112-139     -> gp_tc
+ *   - Graph CSR fields  include/graph.h:9-15                 -> gp_graph_upload / gp_graph_free
+ *   - Schedule runtime tables  include/schedule.h:23-37,56-58 -> gp_sched_flat
+ *   - Schedule constructor  src/schedule.cpp:9-253           -> gp_schedule_build (host only)
+ *   - DataLoader::load_data  src/dataloader.cpp:8-190        -> gp_csr_load (host only)
+ *   - MotifGenerator::generate  src/motif_generator.cpp:39   -> gp_motifs (host only)
+ */
+#ifndef GRAPHPI_AMD_H
+#define GRAPHPI_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_API __attribute__((visibility("default")))
+
+/* ---- status codes ---- */
+#define GP_OK 0
+#define GP_EINVAL (-1)     /* bad argument */
+#define GP_ENOMEM (-2)     /* allocation failed or over the configured budget */
+#define GP_ERANGE (-3)     /* size limit exceeded (pattern too large, ...) */
+#define GP_EIO (-4)        /* file missing or malformed */
+#define GP_EPATTERN (-5)   /* pattern has a disconnected prefix (is_pattern_valid == false) */
+#define GP_ENODEV (-6)     /* no HIP device */
+
+/* ---- limits ---- */
+#define GP_MAX_SIZE 8                       /* pattern vertices */
+#define GP_MAX_PREFIX 28                    /* size*(size-1)/2 */
+#define GP_MAX_RESTRICT 28
+#define GP_MAX_IEP_K 6                      /* trailing independent set counted by IEP */
+#define GP_MAX_IEP_GROUPS 203               /* Bell(6) */
+
+/*
+ * Flattened Schedule: a POD copy of the tables Graph::pattern_matching reads
+ * (include/schedule.h:23-37,56-58; built by src/schedule.cpp:329-398,1114-1210,
+ * 1746-1764).  Unused tail entries are -1 / 0.
+ *   father[p]          get_father_prefix_id(p), -1 = the neighbour list itself
+ *   last[d], next[p]   prefixes whose last element is depth d (linked list)
+ *   loop_set[d]        get_loop_set_prefix_id(d)
+ *   restrict_*         restriction lists keyed by depth: vertex at depth d must
+ *                      be < embedding[restrict_index[r]] for r in list(d)
+ *   iep_num            in_exclusion_optimize_num (0 = off)
+ *   iep_block[g][i]    block id (0-based, in order of first appearance) of
+ *                      trailing vertex i in partition g; iep_val[g] its weight
+ *   redundancy         get_in_exclusion_optimize_redundancy()
+ */
+typedef struct gp_sched_flat {
+    int32_t size;
+    int32_t total_prefix_num;
+    int32_t father[GP_MAX_PREFIX];
+    int32_t next[GP_MAX_PREFIX];
+    int32_t last[GP_MAX_SIZE];
+    int32_t loop_set[GP_MAX_SIZE];
+    int32_t total_restrict_num;
+    int32_t restrict_last[GP_MAX_SIZE];
+    int32_t restrict_next[GP_MAX_RESTRICT];
+    int32_t restrict_index[GP_MAX_RESTRICT];
+    int32_t iep_num;
+    int32_t iep_group_num;
+    int8_t iep_block[GP_MAX_IEP_GROUPS][GP_MAX_SIZE];
+    int64_t iep_val[GP_MAX_IEP_GROUPS];
+    int64_t redundancy;
+    int32_t adj[GP_MAX_SIZE * GP_MAX_SIZE]; /* schedule-ordered adjacency matrix */
+    int32_t restrict_pair_num;             /* Schedule::restrict_pair after pruning */
+    int32_t restrict_pair[GP_MAX_RESTRICT][2];
+    int32_t multiplicity;                  /* |Aut(pattern)| */
+} gp_sched_flat;
+
+/* Counting options.  Zero-initialise and set what you need. */
+typedef struct gp_tuning {
+    int32_t waves_per_cu;   /* persistent waves per CU; 0 = default */
+    int32_t chunk;          /* depth-1 edge tasks per work fetch; 0 = default */
+    int32_t lds_stage;      /* max father-set elements staged in LDS; 0 = default */
+    int32_t no_truncate;    /* 1 = disable restriction-bound truncation */
+    int32_t collect_balg;   /* 1 = run the reference-faithful loop nest and count B_alg */
+    int32_t reserved[3];
+} gp_tuning;
+
+/* Per-call statistics (all optional outputs). */
+typedef struct gp_stats {
+    double kernel_ms;       /* hipEvent time of the counting kernel(s) */
+    uint64_t balg_bytes;    /* algorithmic bytes (collect_balg=1 only) */
+    uint64_t tasks;         /* depth-1 tasks processed */
+    int32_t waves;          /* persistent waves launched */
+    int32_t kernel_variant; /* which kernel ran */
+} gp_stats;
+
+typedef struct gp_graph gp_graph; /* device-resident CSR (opaque) */
+
+/* ---- device side ---- */
+
+/* Copy a host CSR (include/graph.h:9-15 layout: vertex[v_cnt+1], edge[e_cnt],
+ * e_cnt directed, each list sorted ascending) to the current HIP device. */
+GP_API int gp_graph_upload(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
+                           uint32_t e_cnt, gp_graph** out);
+GP_API int gp_graph_free(gp_graph* g);
+/* Bytes of HBM the handle holds. */
+GP_API int gp_graph_bytes(const gp_graph* g, uint64_t* bytes);
+
+/* Count one flattened schedule over an uploaded graph.  *raw_sum receives the
+ * unsigned 64-bit two's-complement sum of the per-root counts (before the
+ * division by sched->redundancy); *count receives (int64)raw_sum / redundancy,
+ * which is what Graph::pattern_matching returns (src/graph.cpp:244).  Either
+ * pointer may be NULL, as may tuning and stats.  Synchronous on the caller's
+ * thread; uses its own HIP stream. */
+GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
+                    uint64_t* raw_sum, int64_t* count, gp_stats* stats);
+
+/* Triangle count with each triangle counted once (Graph::triangle_counting_mt,
+ * src/graph.cpp:112-139). */
+GP_API int gp_tc(const gp_graph* g, uint64_t* triangles, gp_stats* stats);
+
+/* Number of HIP devices visible (0 when none). */
+GP_API int gp_device_count(int32_t* n);
+
+GP_API const char* gp_last_error(void);
+
+/* ---- host side (no device needed) ---- */
+
+/* Build a Schedule exactly as Schedule(pattern, valid, pm, rt, iep, v_cnt,
+ * e_cnt, tri_cnt) does (src/schedule.cpp:9-253) and flatten it.  adj is the
+ * row-major size x size 0/1 adjacency of the pattern.  *valid mirrors
+ * is_pattern_valid; the call still returns GP_OK when the pattern is invalid. */
+GP_API int gp_schedule_build(int32_t size, const int32_t* adj, int32_t performance_modeling_type,
+                             int32_t restricts_type, int32_t use_in_exclusion_optimize,
+                             int32_t v_cnt, uint32_t e_cnt, int64_t tri_cnt, gp_sched_flat* out,
+                             int32_t* valid);
+
+/* Host CSR owned by the library (free with gp_csr_free). */
+typedef struct gp_csr {
+    int32_t v_cnt;
+    uint32_t e_cnt;      /* directed edge count */
+    int32_t max_degree;
+    int32_t max_intersection_size; /* second-largest degree */
+    int64_t tri_cnt;     /* -1 when unknown */
+    int32_t* edge;
+    uint32_t* vertex;
+} gp_csr;
+
+/* Load a GraphPi text graph ("v_cnt e_cnt" then one undirected edge per line)
+ * with DataLoader::general_load_data semantics (src/dataloader.cpp:20-163):
+ * ids remapped by first appearance, self-loops dropped, duplicates removed,
+ * header counts checked.  oriented_type: 0 keep, 1 high degree first, 2 low
+ * degree first.  binary=1 reads the u32 CSR format of
+ * DataLoader::twitter_load_data (src/dataloader.cpp:165-190). */
+GP_API int gp_csr_load(const char* path, int32_t binary, int32_t oriented_type, gp_csr** out);
+/* Complete graph K_n (DataLoader::load_complete, src/dataloader.cpp:192-262). */
+GP_API int gp_csr_complete(int32_t n, gp_csr** out);
+/* Build from an undirected edge list with the same semantics as writing it as
+ * a GraphPi text file and loading it (self-loops dropped, duplicates removed,
+ * first-appearance remap).  edges holds 2*n_edges int32 endpoints. */
+GP_API int gp_csr_from_edges(const int32_t* edges, uint64_t n_edges, gp_csr** out);
+/* Synthetic graphs.  gen = 0: R-MAT(scale, edge_factor, a, b, c, seed);
+ * gen = 1: Chung-Lu(n = 1<<scale is NOT used; n = p0, avg undirected edges =
+ * p1, Pareto exponent = a, max expected degree = b, seed).  See the generator
+ * source for the exact sampling procedure. */
+GP_API int gp_csr_generate(int32_t gen, int64_t p0, int64_t p1, double a, double b, double c,
+                           uint64_t seed, gp_csr** out);
+/* Write a GraphPi text file (unique undirected edges u<v, ids as in the CSR). */
+GP_API int gp_csr_write_text(const gp_csr* g, const char* path);
+/* Write the binary u32 CSR [v_cnt, e_cnt, max_deg, vertex[v_cnt+1], edge[e_cnt]]. */
+GP_API int gp_csr_write_binary(const gp_csr* g, const char* path);
+/* FNV-1a 64 over vertex[] then edge[] (identifies a CSR bit for bit). */
+GP_API int gp_csr_hash(const gp_csr* g, uint64_t* hash);
+GP_API int gp_csr_free(gp_csr* g);
+
+/* All connected patterns on n vertices in MotifGenerator order
+ * (src/motif_generator.cpp:5-44).  adj_out receives count*n*n int32 entries;
+ * pass NULL to query *count first. */
+GP_API int gp_motifs(int32_t n, int32_t* adj_out, int32_t capacity, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPHPI_AMD_H */
