@@ -39,6 +39,21 @@ extern "C" int gp_schedule_build(int32_t size, const int32_t* adj, int32_t perfo
                 if (i == j) return gp_fail(GP_EINVAL, "pattern has a self-loop");
                 p.add_edge(i, j);
             }
+    // The Schedule constructor (like the reference's, src/schedule.cpp:268-284)
+    // aborts when restriction generation meets a vertex with no earlier
+    // neighbour in the given order; report such input as invalid instead.
+    if (performance_modeling_type == 0) {
+        for (int i = 1; i < size; ++i) {
+            bool linked = false;
+            for (int j = 0; j < i && !linked; ++j) linked = adj[i * size + j] != 0;
+            if (!linked) {
+                if (valid) *valid = 0;
+                std::memset(out, 0, sizeof(*out));
+                out->size = size;
+                return GP_OK;
+            }
+        }
+    }
     bool ok = false;
     Schedule s(p, ok, performance_modeling_type, restricts_type, use_in_exclusion_optimize != 0, v_cnt,
                e_cnt, tri_cnt);

@@ -1,0 +1,225 @@
+"""GPU parity: the HIP counting path against the reference's golden counts and
+the oracle, bit for bit (integer counts, tolerance 0).
+
+Every count here goes through the C ABI (gp_count / gp_tc) on the device."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import graphpi_amd as gp
+import oracle_binding as ob
+from conftest import load_json
+
+pytestmark = pytest.mark.gpu
+
+SETTINGS = [(1, 1, 1), (1, 1, 0), (0, 1, 0), (0, 0, 0), (0, 2, 0)]
+SMALL_PATTERNS = ["edge", "wedge", "triangle", "rectangle", "qg3", "house", "pentagon", "4clique", "5clique",
+                  "tri_edge_tri", "hourglass", "cycle_6_tri"]
+
+
+def _tc(host):
+    return ob.triangle_counting(host)
+
+
+def _sched(host, adj, pm, rt, iep, tri=None):
+    tri = _tc(host) if tri is None else tri
+    s, valid = gp.build_schedule(adj, pm, rt, iep, max(host.v_cnt, 1), max(host.e_cnt, 1), tri)
+    assert valid
+    return s
+
+
+def _random_graph(n, p, seed, hubs=0):
+    rng = np.random.default_rng(seed)
+    a = rng.random((n, n)) < p
+    for h in range(hubs):  # a few high-degree vertices
+        a[h, :] = rng.random(n) < 0.7
+    a = np.triu(a, 1)
+    e = np.argwhere(a)
+    return gp.HostGraph.from_edges(e)
+
+
+@pytest.fixture(scope="module")
+def wiki(wiki_vote_path):
+    h = gp.HostGraph.load_text(wiki_vote_path)
+    return h, gp.DeviceGraph(h)
+
+
+def test_device_present():
+    assert gp.device_count() >= 1
+
+
+def test_wiki_vote_reference_counts(wiki):
+    host, dev = wiki
+    recs = [r for r in load_json("reference_counts.json") if r["graph"] == "wiki-vote"]
+    assert recs
+    assert host.hash() == recs[0]["csr_fnv1a"]
+    for r in recs:
+        a = gp.adj_from_string(len(r["pattern_adj"]) and int(math.isqrt(len(r["pattern_adj"]))), r["pattern_adj"])
+        s, valid = gp.build_schedule(a, r["pm"], r["rt"], r["iep"], r["v_cnt"], r["e_cnt"], r["tri_cnt"])
+        assert valid
+        got = dev.count(s)
+        print(f"wiki-vote {r['pattern']} pm={r['pm']} gpu={got.count} ref={r['count']} "
+              f"gpu_ms={got.stats.kernel_ms:.3f} ref_s={r['time_count']:.3f}", flush=True)
+        assert got.count == r["count"], r["pattern"]
+
+
+def test_wiki_vote_triangles_device(wiki):
+    host, dev = wiki
+    assert dev.triangle_counting() == 608389
+
+
+def test_rmat13_reference_counts(rmat13_path):
+    recs = [r for r in load_json("reference_counts.json") if r["graph"] == "rmat13"]
+    assert recs
+    host = gp.HostGraph.load_text(rmat13_path)
+    assert host.hash() == recs[0]["csr_fnv1a"]
+    dev = gp.DeviceGraph(host)
+    for r in recs:
+        n = int(math.isqrt(len(r["pattern_adj"])))
+        a = gp.adj_from_string(n, r["pattern_adj"])
+        tri = r["tri_cnt"]
+        s, valid = gp.build_schedule(a, r["pm"], r["rt"], r["iep"], r["v_cnt"], r["e_cnt"], tri)
+        assert valid
+        got = dev.count(s)
+        print(f"rmat13 {r['pattern']} ({r['pm']},{r['rt']},{r['iep']}) gpu={got.count} ref={r['count']} "
+              f"gpu_ms={got.stats.kernel_ms:.3f} ref_s={r['time_count']:.3f}", flush=True)
+        assert got.count == r["count"], (r["pattern"], r["pm"], r["rt"], r["iep"])
+    assert dev.triangle_counting() == recs[0]["tri_cnt"]
+
+
+@pytest.mark.parametrize("seed,n,p,hubs", [(1, 40, 0.25, 0), (2, 70, 0.12, 2), (3, 120, 0.05, 3)])
+def test_random_graphs_match_oracle(seed, n, p, hubs):
+    host = _random_graph(n, p, seed, hubs)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    for name in SMALL_PATTERNS:
+        adj = gp.PATTERNS[name]
+        for pm, rt, iep in SETTINGS:
+            s, valid = gp.build_schedule(adj, pm, rt, iep, host.v_cnt, host.e_cnt, tri)
+            if not valid:
+                continue
+            want = ob.pattern_matching(host, s, threads=4)
+            got = dev.count(s)
+            assert (got.raw, got.count) == (want["raw"], want["count"]), (name, pm, rt, iep)
+
+
+def test_faithful_mode_balg_matches_oracle(rmat13_path):
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    tun = gp.Tuning()
+    tun.collect_balg = 1
+    for name in ["triangle", "rectangle", "house", "4clique", "pentagon", "qg3"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        want = ob.pattern_matching(host, s, threads=8)
+        got = dev.count(s, tun)
+        assert got.count == want["count"], name
+        assert got.stats.balg_bytes == want["balg"], name
+
+
+def test_no_truncation_same_counts(rmat13_path):
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    tun = gp.Tuning()
+    tun.no_truncate = 1
+    for name in ["4clique", "5clique", "house", "tri_edge_tri"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        assert dev.count(s, tun).raw == dev.count(s).raw, name
+
+
+@pytest.mark.parametrize("wpc,chunk", [(1, 1), (4, 3), (32, 64)])
+def test_launch_shapes_same_counts(rmat13_path, wpc, chunk):
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    tun = gp.Tuning()
+    tun.waves_per_cu = wpc
+    tun.chunk = chunk
+    for name in ["house", "4clique"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        assert dev.count(s, tun).count == dev.count(s).count
+
+
+def _perm_count(n, k):
+    return math.factorial(n) // math.factorial(n - k)
+
+
+def _ext(n, pairs):
+    return sum(1 for perm in itertools.permutations(range(n)) if all(perm[b] < perm[a] for a, b in pairs))
+
+
+def test_complete_graph_closed_form():
+    # On K_n every candidate set is "all unused vertices": the raw sum is
+    # C(n, k) * L(pairs checked by the loops) with L the number of orderings
+    # obeying them; the reference divides by redundancy = L(front) // L(all)
+    # (integer division, src/schedule.cpp:1762).  Where that ratio is exact
+    # the result is the true embedding count n!/(n-k)!/|Aut|.
+    for n in (5, 8, 12):
+        host = gp.HostGraph.complete(n)
+        dev = gp.DeviceGraph(host)
+        for name in ["triangle", "rectangle", "house", "pentagon", "4clique", "5clique", "hourglass"]:
+            adj = gp.PATTERNS[name]
+            k = adj.shape[0]
+            if k > n:
+                continue
+            s, _ = gp.build_schedule(adj, 1, 1, 1, host.v_cnt, host.e_cnt, math.comb(n, 3))
+            d = s.to_dict()
+            pairs = [tuple(p) for p in d["restrict_pair"]]
+            front = [p for p in pairs if s.iep_num < 2 or p[1] < k - s.iep_num]
+            lf, la = _ext(k, front), _ext(k, pairs)
+            assert s.redundancy == lf // la
+            got = dev.pattern_matching(s)
+            assert got == math.comb(n, k) * lf // s.redundancy, (n, name)
+            if lf % la == 0:
+                assert got == _perm_count(n, k) // s.multiplicity, (n, name)
+
+
+def test_complete_bipartite_closed_form():
+    a, b = 7, 9
+    edges = [(i, a + j) for i in range(a) for j in range(b)]
+    host = gp.HostGraph.from_edges(edges)
+    dev = gp.DeviceGraph(host)
+    s = _sched(host, gp.PATTERNS["rectangle"], 1, 1, 1)
+    assert dev.pattern_matching(s) == math.comb(a, 2) * math.comb(b, 2)
+    assert dev.triangle_counting() == 0
+
+
+def test_cycles_and_stars():
+    c5 = gp.HostGraph.from_edges([(i, (i + 1) % 5) for i in range(5)])
+    d5 = gp.DeviceGraph(c5)
+    assert d5.pattern_matching(_sched(c5, gp.PATTERNS["pentagon"], 1, 1, 1)) == 1
+    c7 = gp.HostGraph.from_edges([(i, (i + 1) % 7) for i in range(7)])
+    d7 = gp.DeviceGraph(c7)
+    assert d7.pattern_matching(_sched(c7, gp.PATTERNS["pentagon"], 1, 1, 1)) == 0
+    star = gp.HostGraph.from_edges([(0, i) for i in range(1, 40)])
+    ds = gp.DeviceGraph(star)
+    assert ds.pattern_matching(_sched(star, gp.PATTERNS["wedge"], 1, 1, 1)) == math.comb(39, 2)
+
+
+def test_empty_and_tiny_graphs():
+    empty = gp.HostGraph.from_edges(np.zeros((0, 2), dtype=np.int32))
+    assert empty.v_cnt == 0
+    de = gp.DeviceGraph(empty)
+    s, _ = gp.build_schedule(gp.PATTERNS["triangle"], 0, 1, 0, 1, 1, 0)
+    assert de.pattern_matching(s) == 0
+    one = gp.HostGraph.from_edges([(3, 9)])
+    d1 = gp.DeviceGraph(one)
+    assert d1.pattern_matching(s) == 0
+    se, _ = gp.build_schedule(gp.PATTERNS["edge"], 0, 1, 0, 2, 2, 0)
+    assert d1.pattern_matching(se) == 1
+
+
+def test_bad_schedule_is_rejected():
+    host = gp.HostGraph.complete(5)
+    dev = gp.DeviceGraph(host)
+    s, _ = gp.build_schedule(gp.PATTERNS["triangle"], 0, 1, 0, 5, 20, 10)
+    s.redundancy = 0
+    with pytest.raises(gp.GPError):
+        dev.count(s)
+    s2, _ = gp.build_schedule(gp.PATTERNS["triangle"], 0, 1, 0, 5, 20, 10)
+    s2.loop_set[2] = 27
+    with pytest.raises(gp.GPError):
+        dev.count(s2)
