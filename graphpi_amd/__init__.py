@@ -93,7 +93,8 @@ class Tuning(ctypes.Structure):
         ("lds_stage", ctypes.c_int32),
         ("no_truncate", ctypes.c_int32),
         ("collect_balg", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("no_batch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

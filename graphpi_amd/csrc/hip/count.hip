@@ -37,6 +37,8 @@
 #include "schedule.h"
 
 #define WAVE 64
+#define GP_HASH_SLOTS 2048      // LDS hash entries per wave for the batched leaf level
+#define GP_MAX_BATCH_BLOCKS 15  // leaf blocks the batched level handles
 
 namespace {
 
@@ -53,6 +55,7 @@ struct KArgs {
     int64_t slot_stride;
     int64_t wave_stride;
     int64_t tmp_stride;
+    int64_t bslot_stride;
     unsigned long long* out; // [0] raw sum, [1] B_alg bytes, [2] tasks taken
 };
 
@@ -135,6 +138,82 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
     return total;
 }
 
+// ---- wave utilities ----
+
+// exclusive prefix sum over the wave; *total receives the sum
+__device__ __forceinline__ int32_t wave_excl_scan(int32_t x, int32_t* total) {
+    const int lane = lane_id();
+    int32_t v = x;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        int32_t y = __shfl_up(v, off);
+        if (lane >= off) v += y;
+    }
+    *total = __shfl(v, WAVE - 1);
+    return v - x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        uint32_t l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+        uint64_t s = ((uint64_t)hi << 32 | lo) + ((uint64_t)h2 << 32 | l2);
+        lo = (uint32_t)s;
+        hi = (uint32_t)(s >> 32);
+    }
+    return (uint64_t)hi << 32 | lo;
+}
+
+// lane-private lower bound (first index with a[i] >= x)
+__device__ __forceinline__ int32_t lower_bound_lane(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+    int32_t lo = 0, len = n;
+    while (len > 0) {
+        int32_t half = len >> 1;
+        int32_t mid = lo + half;
+        bool right = a[mid] < x;
+        lo = right ? mid + 1 : lo;
+        len = right ? len - half - 1 : half;
+    }
+    return lo;
+}
+
+// open-addressing hash set of vertex ids in LDS (empty slot = -1)
+__device__ __forceinline__ uint32_t hash_slot(int32_t x, int lg) {
+    return ((uint32_t)x * 2654435761u) >> (32 - lg);
+}
+
+__device__ void hash_fill(int32_t* tab, int lg, const int32_t* __restrict__ g, int32_t n) {
+    const int lane = lane_id();
+    const int cap = 1 << lg;
+    for (int i = lane; i < cap; i += WAVE) tab[i] = -1;
+    __syncthreads();
+    for (int32_t base = 0; base < n; base += WAVE) {
+        int32_t i = base + lane;
+        if (i < n) {
+            int32_t x = g[i];
+            uint32_t h = hash_slot(x, lg);
+            for (;;) {
+                int32_t old = atomicCAS(&tab[h], -1, x);
+                if (old == -1 || old == x) break;
+                h = (h + 1) & (cap - 1);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool hash_has(const int32_t* tab, int lg, int32_t x) {
+    const uint32_t mask = (1u << lg) - 1;
+    uint32_t h = hash_slot(x, lg);
+    for (;;) {
+        int32_t y = tab[h];
+        if (y == x) return true;
+        if (y == -1) return false;
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ int ilog2_ceil(int32_t x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
+
 template <bool FAITHFUL>
 __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
@@ -144,13 +223,24 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     __shared__ int32_t s_ll[GP_MAX_SIZE];
     __shared__ int32_t s_li[GP_MAX_SIZE];
     __shared__ int32_t s_blk[GP_MAX_BLOCKS];
+    // batched leaf level
+    __shared__ int32_t s_hash[GP_HASH_SLOTS];
+    __shared__ int32_t s_bc[GP_MAX_BATCH_BLOCKS][WAVE];
+    __shared__ const int32_t* s_bg[GP_MAX_BATCH_BLOCKS];
+    __shared__ int32_t s_bn[GP_MAX_BATCH_BLOCKS];
+    __shared__ int32_t s_bh[GP_MAX_BATCH_BLOCKS];  // LDS hash offset, -1 = not staged
+    __shared__ int32_t s_blg[GP_MAX_BATCH_BLOCKS];
+    __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kind 0)
+    __shared__ int32_t s_off[WAVE + 1];
 
     const KPlan& P = *a.plan;
     const int lane = lane_id();
     const int n = P.n, L = P.leaf, K = P.k;
     const int32_t v_cnt = a.v_cnt;
     int32_t* wbase = a.scratch + (int64_t)blockIdx.x * a.wave_stride;
+    int32_t* bbase = wbase + (int64_t)P.nslots * a.slot_stride + 2 * a.tmp_stride;
     uint64_t acc = 0, balg = 0, taken = 0;
+    uint64_t lacc = 0; // per-lane partial sums of the batched leaf level
 
     auto bound = [&](uint32_t mask) -> int32_t {
         int32_t b = v_cnt;
@@ -187,11 +277,8 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     };
     // build the prefixes whose last vertex is depth d; false when one is empty
     auto build_level = [&](int d, const int32_t* nv, int32_t deg) -> bool {
-        for (int i = 0; i < P.nbuild[d]; ++i) {
-            const int p = P.build[d][i];
-            if (!FAITHFUL && p == P.fused_prefix) continue; // counted by the leaf
-            if (build_prefix(p, nv, deg) == 0) return false;
-        }
+        for (int i = 0; i < P.nbuild[d]; ++i)
+            if (build_prefix(P.build[d][i], nv, deg) == 0) return false;
         return true;
     };
     // count of the common elements of the block's sets, minus embedded ones
@@ -223,15 +310,11 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         }
         return total;
     };
-    // count at depth L with emb[0:L) placed; nv/deg = N(emb[L-1])
-    auto leaf = [&](const int32_t* nv, int32_t deg) -> uint64_t {
+    // count at depth L with emb[0:L) placed and every set built (one candidate)
+    auto leaf = [&]() -> uint64_t {
         if (K == 0) {
             const int q = P.loop_set[L];
             const int32_t b = bound(P.rmask[L]);
-            if (!FAITHFUL && P.fused) {
-                const int f = P.father[q];
-                return (uint64_t)(int64_t)intersect<true>(s_ptr[f], s_len[f], nv, deg, b, nullptr, s_emb, L);
-            }
             const int32_t* S = s_ptr[q];
             int32_t c = s_len[q];
             if (P.rmask[L]) c = wave_lower_bound(S, c, b);
@@ -309,6 +392,207 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         s_li[d] = 0;
     };
 
+    // The last loop level (depth d = L-1) as one batch: every candidate v of
+    // the loop set gets its leaf value from block counts (see KPlan); the
+    // sets shared by all candidates are intersected once and staged in an
+    // LDS hash, and the candidates' neighbour lists are streamed flattened
+    // across the 64 lanes.  Candidates whose list is far longer than the
+    // staged sets are handled by probing the sets into the list instead.
+    auto leaf_level = [&](int d) {
+        const int32_t ncand = s_ll[d];
+        if (ncand <= 0) return;
+        const int32_t* cand = s_lp[d];
+        const int nb = P.nblocks;
+        // 1. the fixed set G of every block
+        int32_t smax = -1;       // largest element of any streamed set
+        int nstream = 0;
+        int stream_cost_w = 0;   // per streamed element, LDS or global lookups
+        int32_t stream_total = 0;
+        int hoff = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int kind = P.bl_kind[b];
+            const int32_t* g = nullptr;
+            int32_t gn = 0;
+            if (kind != 2) {
+                const int ns = P.bl_nsrc[b];
+                const int p0 = P.bl_src[b][0];
+                g = s_ptr[p0];
+                gn = s_len[p0];
+                if (ns >= 2) {
+                    int32_t* out = bbase + (int64_t)P.bl_slot[b] * a.bslot_stride;
+                    int32_t* spare = bbase + (int64_t)P.nbslots * a.bslot_stride;
+                    for (int i = 1; i < ns; ++i) {
+                        const int pi = P.bl_src[b][i];
+                        // alternate so the last intersection lands in 'out'
+                        int32_t* dst = ((ns - 1 - i) & 1) ? spare : out;
+                        gn = intersect<false>(g, gn, s_ptr[pi], s_len[pi], INT_MAX, dst, nullptr, 0);
+                        __syncthreads();
+                        g = dst;
+                    }
+                }
+            }
+            s_bg[b] = g;
+            s_bn[b] = gn;
+            s_bh[b] = -1;
+            s_blg[b] = 0;
+            if (kind == 0) {
+                bool hit = lane < d && gn > 0 && contains(g, gn, s_emb[lane]);
+                s_bemb[b] = (int32_t)__ballot(hit);
+            } else if (kind == 1) {
+                ++nstream;
+                stream_total += gn;
+                if (gn > 0) smax = max(smax, g[gn - 1]);
+                const int lg = max(6, ilog2_ceil(2 * gn));
+                if (gn > 0 && hoff + (1 << lg) <= GP_HASH_SLOTS) {
+                    hash_fill(s_hash + hoff, lg, g, gn);
+                    s_bh[b] = hoff;
+                    s_blg[b] = lg;
+                    hoff += 1 << lg;
+                    stream_cost_w += 1;
+                } else {
+                    stream_cost_w += ilog2_ceil(gn + 1) + 1;
+                }
+            }
+        }
+        const bool plain = K == 0;
+        const uint32_t leaf_mask = plain ? P.rmask[L] : 0;
+        // 2. candidates in batches of 64
+        for (int32_t base = 0; base < ncand; base += WAVE) {
+            const int32_t ci = base + lane;
+            bool valid = ci < ncand;
+            const int32_t v = valid ? cand[ci] : -1;
+            for (int j = 0; j < d; ++j) valid = valid && v != s_emb[j];
+            int32_t nvs = 0, deg = 0;
+            if (valid) {
+                const uint32_t l = a.vertex[v];
+                nvs = (int32_t)l;
+                deg = (int32_t)(a.vertex[v + 1] - l);
+            }
+            const int32_t* nv = a.edge + nvs;
+            // leaf bound with this candidate placed at depth d
+            int32_t b = INT_MAX;
+            if (leaf_mask) {
+                b = v_cnt;
+                for (int j = 0; j < n; ++j)
+                    if (leaf_mask >> j & 1) b = min(b, j == d ? v : s_emb[j]);
+            }
+            // streamed part of N(v): elements below b and below max(G)+1
+            const int32_t scut = smax < 0 ? -1 : min(b, smax + 1);
+            int32_t work = 0;
+            bool probe = false;
+            if (valid && nstream > 0 && scut >= 0) {
+                int32_t cut = deg <= 16 ? deg : lower_bound_lane(nv, deg, scut);
+                // stream the list, or probe the sets into it when cheaper
+                const int64_t stream_cost = (int64_t)cut * stream_cost_w;
+                const int64_t probe_cost = (int64_t)stream_total * (ilog2_ceil(cut + 1) + 1) * 2;
+                probe = cut > 64 && probe_cost < stream_cost;
+                work = probe ? 0 : cut;
+            }
+            for (int b2 = 0; b2 < nb; ++b2)
+                if (P.bl_kind[b2] == 1) s_bc[b2][lane] = 0;
+            int32_t total = 0;
+            const int32_t off = wave_excl_scan(work, &total);
+            s_off[lane] = off;
+            if (lane == WAVE - 1) s_off[WAVE] = total;
+            __syncthreads();
+            for (int32_t f0 = 0; f0 < total; f0 += WAVE) {
+                const int32_t f = f0 + lane;
+                const bool act = f < total;
+                // owner: last lane whose offset is <= f
+                int lo = 0, hi = WAVE;
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= f) lo = mid; else hi = mid;
+                }
+                const int owner = act ? lo : WAVE - 1;
+                const int32_t o_nvs = __shfl(nvs, owner);
+                const int32_t o_off = __shfl(off, owner);
+                const int32_t o_cut = __shfl(scut, owner);
+                const int32_t x = act ? a.edge[o_nvs + (f - o_off)] : -1;
+                bool keep = act && x < o_cut;
+                for (int j = 0; j < d; ++j) keep = keep && x != s_emb[j];
+                const int prev = __shfl_up(owner, 1);
+                const bool start = act && (lane == 0 || prev != owner);
+                const uint64_t starts = __ballot(start || !act);
+                const uint64_t above = lane == WAVE - 1 ? 0ull : starts & (~0ull << (lane + 1));
+                const int next = above ? __ffsll((long long)above) - 1 : WAVE;
+                const uint64_t seg = (next == WAVE ? ~0ull : ((1ull << next) - 1)) & (~0ull << lane);
+                for (int b2 = 0; b2 < nb; ++b2) {
+                    if (P.bl_kind[b2] != 1) continue;
+                    bool hit = false;
+                    if (keep) {
+                        const int h = s_bh[b2];
+                        hit = h >= 0 ? hash_has(s_hash + h, s_blg[b2], x) : contains(s_bg[b2], s_bn[b2], x);
+                    }
+                    const uint64_t hm = __ballot(hit);
+                    if (start) {
+                        const int c = popc64(hm & seg);
+                        if (c) atomicAdd(&s_bc[b2][owner], c);
+                    }
+                }
+            }
+            __syncthreads();
+            // candidates whose lists are probed
+            uint64_t pm = __ballot(probe);
+            while (pm) {
+                const int j = __ffsll((long long)pm) - 1;
+                pm &= pm - 1;
+                const int32_t* pv = a.edge + __shfl(nvs, j);
+                const int32_t pd = __shfl(deg, j);
+                const int32_t pc = __shfl(scut, j);
+                for (int b2 = 0; b2 < nb; ++b2) {
+                    if (P.bl_kind[b2] != 1) continue;
+                    const int32_t c = intersect<true>(s_bg[b2], s_bn[b2], pv, pd, pc, nullptr, s_emb, d);
+                    if (lane == j) s_bc[b2][j] = c;
+                }
+            }
+            __syncthreads();
+            // 3. leaf value of each candidate
+            if (valid) {
+                int32_t cnt[GP_MAX_BATCH_BLOCKS];
+#pragma unroll
+                for (int b2 = 0; b2 < GP_MAX_BATCH_BLOCKS; ++b2) {
+                    cnt[b2] = 0;
+                    if (b2 >= nb) continue;
+                    const int kind = P.bl_kind[b2];
+                    if (kind == 1) {
+                        cnt[b2] = s_bc[b2][lane];
+                    } else if (kind == 2) {
+                        // |N(v) ∩ [0, b) \ emb|
+                        int32_t c = b == INT_MAX ? deg : lower_bound_lane(nv, deg, b);
+                        for (int j = 0; j < d; ++j)
+                            if (s_emb[j] < b && contains(nv, c, s_emb[j])) --c;
+                        cnt[b2] = c;
+                    } else {
+                        // |G ∩ [0, b) \ emb|, emb including v itself
+                        const int32_t* g = s_bg[b2];
+                        const int32_t gn = s_bn[b2];
+                        int32_t c = b == INT_MAX ? gn : lower_bound_lane(g, gn, b);
+                        const uint32_t em = (uint32_t)s_bemb[b2];
+                        for (int j = 0; j < d; ++j)
+                            if ((em >> j & 1) && s_emb[j] < b) --c;
+                        if (v < b && contains(g, gn, v)) --c;
+                        cnt[b2] = c;
+                    }
+                }
+                uint64_t sum = 0;
+                for (int g = 0; g < P.ngroups; ++g) {
+                    uint64_t val = (uint64_t)P.group_val[g];
+                    for (int j = 0; j < P.group_nb[g]; ++j) {
+                        int32_t c = 0;
+#pragma unroll
+                        for (int b2 = 0; b2 < GP_MAX_BATCH_BLOCKS; ++b2)
+                            if (b2 == P.group_blk[g][j]) c = cnt[b2];
+                        val *= (uint64_t)(int64_t)c;
+                    }
+                    sum += val;
+                }
+                lacc += sum;
+            }
+            __syncthreads();
+        }
+    };
+
     for (;;) {
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
@@ -332,7 +616,12 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             const int32_t d0 = (int32_t)(a.vertex[v0 + 1] - l0);
             for (int i = 0; i < P.nbuild[0]; ++i) build_prefix(P.build[0][i], n0, d0);
             if (P.root_mode) {
-                acc += leaf(n0, d0);
+                if (L == 1) {
+                    acc += leaf();
+                } else {
+                    enter(1);
+                    leaf_level(1);
+                }
                 continue;
             }
             s_emb[1] = v1;
@@ -342,10 +631,15 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             const int32_t d1 = (int32_t)(a.vertex[v1 + 1] - l1);
             if (!build_level(1, n1, d1)) continue;
             if (L == 2) {
-                acc += leaf(n1, d1);
+                acc += leaf();
                 continue;
             }
-            // depth-first walk over depths 2 .. L-1
+            if (P.batched && L == 3) {
+                enter(2);
+                leaf_level(2);
+                continue;
+            }
+            // depth-first walk over depths 2 .. L-1 (L-2 when batched)
             int d = 2;
             enter(d);
             for (;;) {
@@ -366,8 +660,13 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 const int32_t* nv = a.edge + lv;
                 const int32_t dv = (int32_t)(a.vertex[v + 1] - lv);
                 if (!build_level(d, nv, dv)) continue;
-                if (d == L - 1) {
-                    acc += leaf(nv, dv);
+                if (P.batched && d == L - 2) {
+                    enter(L - 1);
+                    leaf_level(L - 1);
+                    continue;
+                }
+                if (!P.batched && d == L - 1) {
+                    acc += leaf();
                     continue;
                 }
                 ++d;
@@ -375,8 +674,9 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             }
         }
     }
+    const uint64_t lsum = wave_sum_u64(lacc);
     if (lane == 0) {
-        atomicAdd(&a.out[0], (unsigned long long)acc);
+        atomicAdd(&a.out[0], (unsigned long long)(acc + lsum));
         if (FAITHFUL) atomicAdd(&a.out[1], (unsigned long long)balg);
         atomicAdd(&a.out[2], (unsigned long long)taken);
     }
@@ -498,7 +798,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
               int64_t* count, gp_stats* stats) {
     const bool faithful = tuning && tuning->collect_balg;
     KPlan kp;
-    int rc = gp_make_kplan(sched, faithful, &kp);
+    int rc = gp_make_kplan(sched, faithful, !(tuning && tuning->no_batch), &kp);
     if (rc != GP_OK) return rc;
     if (tuning && tuning->no_truncate) {
         std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
@@ -514,8 +814,12 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     // neighbour list of a vertex other than the root, so at most the
     // second-largest degree), plus two IEP temporaries in faithful mode
     const int64_t slot_stride = ((int64_t)std::max(g->max_deg2, 1) + 63) / 64 * 64;
-    const int64_t tmp_stride = faithful && kp.k ? ((int64_t)std::max(g->max_deg, 1) + 63) / 64 * 64 : 0;
-    const int64_t wave_stride = kp.nslots * slot_stride + 2 * tmp_stride;
+    const int64_t deg_stride = ((int64_t)std::max(g->max_deg, 1) + 63) / 64 * 64;
+    const int64_t tmp_stride = faithful && kp.k ? deg_stride : 0;
+    // batched leaf level: intersections of a block's source sets (plus one
+    // spare for three or more sources); each is bounded by a neighbour list
+    const int64_t bslot_stride = kp.batched && kp.nbslots ? deg_stride : 0;
+    const int64_t wave_stride = kp.nslots * slot_stride + 2 * tmp_stride + (kp.nbslots + 1) * bslot_stride;
     const size_t scratch_bytes = (size_t)std::max<int64_t>(waves * wave_stride, 64) * sizeof(int32_t);
 
     CallCtx* c = nullptr;
@@ -534,6 +838,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     a.slot_stride = slot_stride;
     a.wave_stride = wave_stride;
     a.tmp_stride = tmp_stride;
+    a.bslot_stride = bslot_stride;
     a.out = c->d_cnt + 1;
     unsigned long long host_out[4] = {0, 0, 0, 0};
     hipError_t e = hipMemcpyAsync(c->d_plan, &kp, sizeof(KPlan), hipMemcpyHostToDevice, c->stream);

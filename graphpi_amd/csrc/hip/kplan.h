@@ -32,7 +32,21 @@ struct KPlan {
     uint8_t group_nb[GP_MAX_IEP_GROUPS];
     uint8_t group_blk[GP_MAX_IEP_GROUPS][GP_MAX_IEP_K];
     int64_t group_val[GP_MAX_IEP_GROUPS];
+    // batched last loop level: the candidates of depth leaf-1 are processed
+    // together; each leaf block is one of
+    //   kind 0: |G \ emb|            G fixed for the whole batch
+    //   kind 1: |G ∩ N(v) \ emb|     G fixed, N(v) the candidate's list
+    //   kind 2: |N(v) \ emb|
+    // where G is the intersection of the block's source prefixes (a scratch
+    // slot holds it when there are two or more).  A plain leaf is one block
+    // with the restriction bound rmask[leaf]; an IEP leaf uses the groups.
+    int32_t batched;
+    int32_t nbslots;
+    uint8_t bl_kind[GP_MAX_BLOCKS];
+    uint8_t bl_nsrc[GP_MAX_BLOCKS];
+    int8_t bl_src[GP_MAX_BLOCKS][GP_MAX_IEP_K];
+    int8_t bl_slot[GP_MAX_BLOCKS];
 };
 
 // Returns GP_OK or an error code (with gp_fail message) for unsupported input.
-int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* out);
+int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, KPlan* out);

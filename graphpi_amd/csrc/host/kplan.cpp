@@ -38,7 +38,7 @@ uint32_t restrict_mask(uint32_t m, int dp, const uint8_t* rmask) {
 
 } // namespace
 
-int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* kp) {
+int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, KPlan* kp) {
     std::memset(kp, 0, sizeof(*kp));
     const int n = s->size;
     if (n < 2 || n > GP_MAX_SIZE) return gp_fail(GP_ERANGE, "pattern size must be 2..8");
@@ -51,6 +51,10 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* kp) {
     kp->k = k;
     kp->leaf = k ? n - k : n - 1;
     kp->nprefix = P;
+    // The batched level tracks at most 15 leaf blocks (IEP width <= 4).  A
+    // leaf at depth 2 stays per-edge: batching it would make a hub root one
+    // indivisible task.
+    kp->batched = allow_batch && !faithful && kp->leaf >= 3 && k <= 4;
     kp->root_mode = kp->leaf == 1;
     for (int p = 0; p < GP_MAX_PREFIX; ++p) {
         kp->father[p] = -1;
@@ -141,21 +145,15 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* kp) {
         }
     }
 
-    // ---- fused plain leaf ----
     kp->fused = 0;
     kp->fused_prefix = -1;
-    if (!faithful && k == 0 && kp->leaf >= 2) {
-        int q = kp->loop_set[kp->leaf];
-        if (kp->depth_of[q] == kp->leaf - 1 && kp->nbuild[kp->leaf - 1] == 1 && kp->father[q] >= 0) {
-            kp->fused = 1;
-            kp->fused_prefix = q;
-        }
-    }
 
     // ---- scratch slots ----
+    // in batched mode the sets of depth leaf-1 are never built: the leaf
+    // blocks read their fathers and the candidate's neighbour list instead
     int slots = 0;
     for (int p = 0; p < P; ++p)
-        if (kp->father[p] >= 0 && p != kp->fused_prefix) kp->slot[p] = (int8_t)slots++;
+        if (kp->father[p] >= 0 && !(kp->batched && kp->depth_of[p] == kp->leaf - 1)) kp->slot[p] = (int8_t)slots++;
     kp->nslots = slots;
 
     // ---- IEP tables ----
@@ -183,6 +181,43 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, KPlan* kp) {
                 kp->group_blk[g][b] = (uint8_t)idx;
             }
             kp->group_val[g] = s->iep_val[g];
+        }
+    }
+
+    // ---- batched leaf blocks ----
+    if (kp->batched) {
+        const int d = kp->leaf - 1;
+        auto add_source = [&](int b, int p) {
+            for (int i = 0; i < kp->bl_nsrc[b]; ++i)
+                if (kp->bl_src[b][i] == p) return;
+            kp->bl_src[b][kp->bl_nsrc[b]++] = (int8_t)p;
+        };
+        auto fill_block = [&](int b, uint32_t members) {
+            bool uses_nv = false;
+            for (int i = 0; i < (k ? k : 1); ++i) {
+                if (!(members >> i & 1)) continue;
+                const int q = kp->loop_set[kp->leaf + i];
+                if (kp->depth_of[q] == d) {
+                    uses_nv = true;
+                    if (kp->father[q] >= 0) add_source(b, kp->father[q]);
+                } else {
+                    add_source(b, q);
+                }
+            }
+            kp->bl_kind[b] = (uint8_t)(uses_nv ? (kp->bl_nsrc[b] ? 1 : 2) : 0);
+            kp->bl_slot[b] = -1;
+            if (kp->bl_nsrc[b] >= 2) kp->bl_slot[b] = (int8_t)kp->nbslots++;
+        };
+        if (k == 0) {
+            kp->nblocks = 1;
+            kp->block_mask[0] = 1;
+            kp->ngroups = 1;
+            kp->group_nb[0] = 1;
+            kp->group_blk[0][0] = 0;
+            kp->group_val[0] = 1;
+            fill_block(0, 1);
+        } else {
+            for (int b = 0; b < kp->nblocks; ++b) fill_block(b, kp->block_mask[b]);
         }
     }
     return GP_OK;

@@ -90,7 +90,8 @@ typedef struct gp_tuning {
     int32_t lds_stage;      /* max father-set elements staged in LDS; 0 = default */
     int32_t no_truncate;    /* 1 = disable restriction-bound truncation */
     int32_t collect_balg;   /* 1 = run the reference-faithful loop nest and count B_alg */
-    int32_t reserved[3];
+    int32_t no_batch;       /* 1 = evaluate the last loop level one candidate at a time */
+    int32_t reserved[2];
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */

@@ -119,6 +119,23 @@ def test_faithful_mode_balg_matches_oracle(rmat13_path):
         assert got.stats.balg_bytes == want["balg"], name
 
 
+def test_batched_level_matches_per_candidate(rmat13_path):
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    tun = gp.Tuning()
+    tun.no_batch = 1
+    cases = [(name, st) for name in ["triangle", "wedge", "rectangle", "qg3", "house", "4clique", "5clique"]
+             for st in [(1, 1, 1), (0, 1, 0), (1, 1, 0)]]
+    cases += [("pentagon", (1, 1, 1)), ("cycle_6_tri", (1, 1, 1))]
+    for name, (pm, rt, iep) in cases:
+        s = _sched(host, gp.PATTERNS[name], pm, rt, iep, tri)
+        a, b = dev.count(s), dev.count(s, tun)
+        print(f"{name} ({pm},{rt},{iep}) batched {a.stats.kernel_ms:.2f} ms per-candidate "
+              f"{b.stats.kernel_ms:.2f} ms", flush=True)
+        assert a.raw == b.raw, (name, pm, rt, iep)
+
+
 def test_no_truncation_same_counts(rmat13_path):
     host = gp.HostGraph.load_text(rmat13_path)
     dev = gp.DeviceGraph(host)
