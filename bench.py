@@ -124,7 +124,9 @@ def main():
         for name in cfg["patterns"]:
             log(f"  counting {name} ...")
             out[name] = dev.count(scheds[name])
-            log(f"  {name}: {out[name].count} in {out[name].stats.kernel_ms:.2f} ms")
+            st = out[name].stats
+            log(f"  {name}: {out[name].count} in {st.kernel_ms:.2f} ms (waves done: half {st.finish_p50_ms:.1f} ms, "
+                f"90% {st.finish_p90_ms:.1f} ms, all {st.finish_max_ms:.1f} ms)")
         return out
 
     for _ in range(args.warmup):

@@ -94,7 +94,8 @@ class Tuning(ctypes.Structure):
         ("no_truncate", ctypes.c_int32),
         ("collect_balg", ctypes.c_int32),
         ("no_batch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("no_pair", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -105,6 +106,9 @@ class Stats(ctypes.Structure):
         ("tasks", ctypes.c_uint64),
         ("waves", ctypes.c_int32),
         ("kernel_variant", ctypes.c_int32),
+        ("finish_p50_ms", ctypes.c_double),
+        ("finish_p90_ms", ctypes.c_double),
+        ("finish_max_ms", ctypes.c_double),
     ]
 
 
@@ -313,6 +317,9 @@ PATTERNS = {
     "edge": adj_from_edges(2, [(0, 1)]),
 }
 PATTERNS["5cycle"] = PATTERNS["pentagon"]
+PATTERNS["6cycle"] = adj_from_edges(6, [(i, (i + 1) % 6) for i in range(6)])
+PATTERNS["4path"] = adj_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+PATTERNS["5path"] = adj_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
 
 
 def build_schedule(adj, performance_modeling_type=1, restricts_type=1, use_in_exclusion_optimize=True,

@@ -39,6 +39,9 @@
 #define WAVE 64
 #define GP_HASH_SLOTS 2048      // LDS hash entries per wave for the batched leaf level
 #define GP_MAX_BATCH_BLOCKS 15  // leaf blocks the batched level handles
+#define GP_PAIR_LG 10           // pair level: count map of 1024 keys + 1024 counts (the hash pool)
+#define GP_PAIR_SLOTS (1 << GP_PAIR_LG)
+static_assert(2 * GP_PAIR_SLOTS <= GP_HASH_SLOTS, "pair map must fit the LDS hash pool");
 
 namespace {
 
@@ -57,6 +60,7 @@ struct KArgs {
     int64_t tmp_stride;
     int64_t bslot_stride;
     unsigned long long* out; // [0] raw sum, [1] B_alg bytes, [2] tasks taken
+    unsigned long long* clocks; // per wave: start, finish (wall clock ticks)
 };
 
 __device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
@@ -232,9 +236,11 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     __shared__ int32_t s_blg[GP_MAX_BATCH_BLOCKS];
     __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kind 0)
     __shared__ int32_t s_off[WAVE + 1];
+    __shared__ int32_t s_off2[WAVE + 1];
 
     const KPlan& P = *a.plan;
     const int lane = lane_id();
+    if (lane == 0) a.clocks[2 * blockIdx.x] = wall_clock64();
     const int n = P.n, L = P.leaf, K = P.k;
     const int32_t v_cnt = a.v_cnt;
     int32_t* wbase = a.scratch + (int64_t)blockIdx.x * a.wave_stride;
@@ -549,21 +555,17 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             __syncthreads();
             // 3. leaf value of each candidate
             if (valid) {
-                int32_t cnt[GP_MAX_BATCH_BLOCKS];
-#pragma unroll
-                for (int b2 = 0; b2 < GP_MAX_BATCH_BLOCKS; ++b2) {
-                    cnt[b2] = 0;
-                    if (b2 >= nb) continue;
+                // kind-1 counts are already in s_bc; fill in the others
+                for (int b2 = 0; b2 < nb; ++b2) {
                     const int kind = P.bl_kind[b2];
-                    if (kind == 1) {
-                        cnt[b2] = s_bc[b2][lane];
-                    } else if (kind == 2) {
+                    if (kind == 2) {
                         // |N(v) ∩ [0, b) \ emb|
-                        int32_t c = b == INT_MAX ? deg : lower_bound_lane(nv, deg, b);
+                        const int32_t cut = b == INT_MAX ? deg : lower_bound_lane(nv, deg, b);
+                        int32_t c = cut;
                         for (int j = 0; j < d; ++j)
-                            if (s_emb[j] < b && contains(nv, c, s_emb[j])) --c;
-                        cnt[b2] = c;
-                    } else {
+                            if (s_emb[j] < b && contains(nv, cut, s_emb[j])) --c;
+                        s_bc[b2][lane] = c;
+                    } else if (kind == 0) {
                         // |G ∩ [0, b) \ emb|, emb including v itself
                         const int32_t* g = s_bg[b2];
                         const int32_t gn = s_bn[b2];
@@ -572,24 +574,178 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                         for (int j = 0; j < d; ++j)
                             if ((em >> j & 1) && s_emb[j] < b) --c;
                         if (v < b && contains(g, gn, v)) --c;
-                        cnt[b2] = c;
+                        s_bc[b2][lane] = c;
                     }
                 }
                 uint64_t sum = 0;
                 for (int g = 0; g < P.ngroups; ++g) {
                     uint64_t val = (uint64_t)P.group_val[g];
-                    for (int j = 0; j < P.group_nb[g]; ++j) {
-                        int32_t c = 0;
-#pragma unroll
-                        for (int b2 = 0; b2 < GP_MAX_BATCH_BLOCKS; ++b2)
-                            if (b2 == P.group_blk[g][j]) c = cnt[b2];
-                        val *= (uint64_t)(int64_t)c;
-                    }
+                    for (int j = 0; j < P.group_nb[g]; ++j) val *= (uint64_t)(int64_t)s_bc[P.group_blk[g][j]][lane];
                     sum += val;
                 }
                 lacc += sum;
             }
             __syncthreads();
+        }
+    };
+
+    // Depths L-2 and L-1 together (KPlan::pair).  The depth-(L-2) candidates
+    // a are taken 64 at a time and split into sub-batches whose sets
+    // G(a) ∩ X fit an LDS count map (key -> number of a whose set holds it);
+    // then the lists N(v) of the depth-(L-1) candidates are streamed once per
+    // sub-batch and each element adds its count.  Pairs with v == a, which the
+    // loop nest skips, are subtracted.  An a whose set is too large for the
+    // map goes through the batched leaf level on its own.
+    auto pair_level = [&]() {
+        const int da = L - 2, dv = L - 1;
+        const int32_t* D = s_lp[da];
+        const int32_t Dn = s_ll[da];
+        if (Dn <= 0) return;
+        const int qc = P.loop_set[dv];
+        const int32_t* C = s_ptr[qc];
+        int32_t Cn = s_len[qc];
+        if (P.rmask[dv]) Cn = wave_lower_bound(C, Cn, bound(P.rmask[dv]));
+        if (Cn <= 0) return;
+        const int32_t b = bound(P.rmask[L]);
+        const int gpf = P.pair_g;
+        const int32_t* gptr = gpf >= 0 ? s_ptr[gpf] : nullptr;
+        const int32_t glen = gpf >= 0 ? s_len[gpf] : 0;
+        int32_t* keys = s_hash;
+        int32_t* cnts = s_hash + GP_PAIR_SLOTS;
+        const int32_t cap_elems = GP_PAIR_SLOTS / 2;
+        for (int32_t abase = 0; abase < Dn; abase += WAVE) {
+            const int32_t ai = abase + lane;
+            bool va = ai < Dn;
+            const int32_t av = va ? D[ai] : -1;
+            for (int j = 0; j < da; ++j) va = va && av != s_emb[j];
+            int32_t as = 0, acut = 0;
+            if (va) {
+                const uint32_t l = a.vertex[av];
+                as = (int32_t)l;
+                const int32_t dg = (int32_t)(a.vertex[av + 1] - l);
+                acut = (b >= v_cnt || dg <= 16) ? dg : lower_bound_lane(a.edge + l, dg, b);
+            }
+            const bool big = va && acut > cap_elems;
+            const int32_t sz = (va && !big) ? acut : 0;
+            int32_t tot = 0;
+            const int32_t ex = wave_excl_scan(sz, &tot);
+            s_off[lane] = ex;
+            __syncthreads();
+            int start = 0;
+            while (start < WAVE) {
+                const int32_t base_e = __shfl(ex, start);
+                const bool fits = lane >= start && (ex + sz - base_e) <= cap_elems;
+                const uint64_t fm = __ballot(fits);
+                int end = fm ? 64 - __clzll((long long)fm) : start + 1;
+                if (end <= start) end = start + 1;
+                const int32_t batch_elems = __shfl(ex + sz, end - 1) - base_e;
+                if (batch_elems > 0) {
+                    for (int i = lane; i < GP_PAIR_SLOTS; i += WAVE) {
+                        keys[i] = -1;
+                        cnts[i] = 0;
+                    }
+                    __syncthreads();
+                    for (int32_t f0 = 0; f0 < batch_elems; f0 += WAVE) {
+                        const int32_t f = f0 + lane;
+                        const bool act = f < batch_elems;
+                        int lo = start, hi = end;
+                        while (hi - lo > 1) {
+                            const int mid = (lo + hi) >> 1;
+                            if (s_off[mid] - base_e <= f) lo = mid; else hi = mid;
+                        }
+                        const int32_t os = __shfl(as, lo);
+                        const int32_t oe = __shfl(ex, lo);
+                        const int32_t x = act ? a.edge[os + (f - (oe - base_e))] : -1;
+                        bool keep = act && x < b;
+                        for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
+                        if (keep && gpf >= 0) keep = contains(gptr, glen, x);
+                        if (keep) {
+                            uint32_t h = hash_slot(x, GP_PAIR_LG);
+                            for (;;) {
+                                const int32_t k = atomicCAS(&keys[h], -1, x);
+                                if (k == -1 || k == x) {
+                                    atomicAdd(&cnts[h], 1);
+                                    break;
+                                }
+                                h = (h + 1) & (GP_PAIR_SLOTS - 1);
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    for (int32_t cbase = 0; cbase < Cn; cbase += WAVE) {
+                        const int32_t ci = cbase + lane;
+                        bool vc = ci < Cn;
+                        const int32_t vv = vc ? C[ci] : -1;
+                        for (int j = 0; j < da; ++j) vc = vc && vv != s_emb[j];
+                        int32_t vs = 0, vcut = 0;
+                        if (vc) {
+                            const uint32_t l = a.vertex[vv];
+                            vs = (int32_t)l;
+                            const int32_t dg = (int32_t)(a.vertex[vv + 1] - l);
+                            vcut = (b >= v_cnt || dg <= 16) ? dg : lower_bound_lane(a.edge + l, dg, b);
+                        }
+                        int32_t vt = 0;
+                        const int32_t vex = wave_excl_scan(vcut, &vt);
+                        s_off2[lane] = vex;
+                        __syncthreads();
+                        for (int32_t f0 = 0; f0 < vt; f0 += WAVE) {
+                            const int32_t f = f0 + lane;
+                            const bool act = f < vt;
+                            int lo = 0, hi = WAVE;
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (s_off2[mid] <= f) lo = mid; else hi = mid;
+                            }
+                            const int32_t os = __shfl(vs, lo);
+                            const int32_t oe = __shfl(vex, lo);
+                            if (act) {
+                                const int32_t x = a.edge[os + (f - oe)];
+                                if (x < b) {
+                                    uint32_t h = hash_slot(x, GP_PAIR_LG);
+                                    for (;;) {
+                                        const int32_t k = keys[h];
+                                        if (k == x) {
+                                            lacc += (uint64_t)cnts[h];
+                                            break;
+                                        }
+                                        if (k == -1) break;
+                                        h = (h + 1) & (GP_PAIR_SLOTS - 1);
+                                    }
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    // pairs v == a
+                    if (lane >= start && lane < end && va && !big && contains(C, Cn, av)) {
+                        const int32_t* na = a.edge + as;
+                        uint64_t own = 0;
+                        for (int32_t i = 0; i < acut; ++i) {
+                            const int32_t x = na[i];
+                            bool keep = x < b;
+                            for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
+                            if (keep && gpf >= 0) keep = contains(gptr, glen, x);
+                            own += keep ? 1 : 0;
+                        }
+                        lacc -= own;
+                    }
+                    __syncthreads();
+                }
+                start = end;
+            }
+            // sets too large for the map
+            uint64_t bm = __ballot(big);
+            while (bm) {
+                const int j = __ffsll((long long)bm) - 1;
+                bm &= bm - 1;
+                const int32_t aj = __shfl(av, j);
+                s_emb[da] = aj;
+                const uint32_t l = a.vertex[aj];
+                if (build_level(da, a.edge + l, (int32_t)(a.vertex[aj + 1] - l))) {
+                    enter(dv);
+                    leaf_level(dv);
+                }
+            }
         }
     };
 
@@ -630,6 +786,11 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             const int32_t* n1 = a.edge + l1;
             const int32_t d1 = (int32_t)(a.vertex[v1 + 1] - l1);
             if (!build_level(1, n1, d1)) continue;
+            if (P.pair && L == 4) {
+                enter(2);
+                pair_level();
+                continue;
+            }
             if (L == 2) {
                 acc += leaf();
                 continue;
@@ -660,6 +821,11 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 const int32_t* nv = a.edge + lv;
                 const int32_t dv = (int32_t)(a.vertex[v + 1] - lv);
                 if (!build_level(d, nv, dv)) continue;
+                if (P.pair && d == L - 3) {
+                    enter(L - 2);
+                    pair_level();
+                    continue;
+                }
                 if (P.batched && d == L - 2) {
                     enter(L - 1);
                     leaf_level(L - 1);
@@ -675,6 +841,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         }
     }
     const uint64_t lsum = wave_sum_u64(lacc);
+    if (lane == 0) a.clocks[2 * blockIdx.x + 1] = wall_clock64();
     if (lane == 0) {
         atomicAdd(&a.out[0], (unsigned long long)(acc + lsum));
         if (FAITHFUL) atomicAdd(&a.out[1], (unsigned long long)balg);
@@ -710,6 +877,8 @@ __global__ void fill_u64_kernel(unsigned long long* p, int n) {
 // ---------------------------------------------------------------------------
 
 struct CallCtx {
+    unsigned long long* d_clocks = nullptr;
+    size_t clocks_cap = 0; // waves
     KPlan* d_plan = nullptr;
     unsigned long long* d_cnt = nullptr; // [0] task counter, [1..3] outputs
     int32_t* d_scratch = nullptr;
@@ -743,6 +912,7 @@ namespace {
 void destroy_ctx(CallCtx* c) {
     if (!c) return;
     if (c->d_plan) (void)hipFree(c->d_plan);
+    if (c->d_clocks) (void)hipFree(c->d_clocks);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     if (c->e0) (void)hipEventDestroy(c->e0);
@@ -798,7 +968,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
               int64_t* count, gp_stats* stats) {
     const bool faithful = tuning && tuning->collect_balg;
     KPlan kp;
-    int rc = gp_make_kplan(sched, faithful, !(tuning && tuning->no_batch), &kp);
+    int rc = gp_make_kplan(sched, faithful, !(tuning && tuning->no_batch), !(tuning && tuning->no_pair), &kp);
     if (rc != GP_OK) return rc;
     if (tuning && tuning->no_truncate) {
         std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
@@ -825,6 +995,17 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     CallCtx* c = nullptr;
     rc = acquire_ctx(g, scratch_bytes, &c);
     if (rc != GP_OK) return rc;
+    if (c->clocks_cap < (size_t)waves) {
+        if (c->d_clocks) (void)hipFree(c->d_clocks);
+        c->d_clocks = nullptr;
+        c->clocks_cap = 0;
+        hipError_t ce = hipMalloc((void**)&c->d_clocks, sizeof(unsigned long long) * 2 * (size_t)waves);
+        if (ce != hipSuccess) {
+            destroy_ctx(c);
+            return gp_fail(GP_ENOMEM, std::string("clock buffer: ") + hipGetErrorString(ce));
+        }
+        c->clocks_cap = (size_t)waves;
+    }
     KArgs a;
     a.plan = c->d_plan;
     a.vertex = g->d_vertex;
@@ -840,7 +1021,9 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     a.tmp_stride = tmp_stride;
     a.bslot_stride = bslot_stride;
     a.out = c->d_cnt + 1;
+    a.clocks = c->d_clocks;
     unsigned long long host_out[4] = {0, 0, 0, 0};
+    std::vector<unsigned long long> clocks(2 * (size_t)waves, 0);
     hipError_t e = hipMemcpyAsync(c->d_plan, &kp, sizeof(KPlan), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt, 4);
@@ -856,6 +1039,9 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     }
     if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && n_tasks > 0)
+        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * clocks.size(), hipMemcpyDeviceToHost,
+                           c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     float ms = 0;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
@@ -875,6 +1061,21 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
         stats->tasks = host_out[3];
         stats->waves = (int32_t)waves;
         stats->kernel_variant = faithful ? 1 : 0;
+        stats->finish_p50_ms = stats->finish_p90_ms = stats->finish_max_ms = 0;
+        if (n_tasks > 0) {
+            int rate_khz = 0;
+            if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
+                rate_khz <= 0)
+                rate_khz = 100000;
+            unsigned long long t0 = ~0ull;
+            for (int64_t w = 0; w < waves; ++w) t0 = std::min(t0, clocks[2 * w]);
+            std::vector<double> fin((size_t)waves);
+            for (int64_t w = 0; w < waves; ++w) fin[w] = (double)(clocks[2 * w + 1] - t0) / rate_khz;
+            std::sort(fin.begin(), fin.end());
+            stats->finish_p50_ms = fin[fin.size() / 2];
+            stats->finish_p90_ms = fin[fin.size() * 9 / 10];
+            stats->finish_max_ms = fin.back();
+        }
     }
     return GP_OK;
 }

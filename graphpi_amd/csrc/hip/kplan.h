@@ -46,7 +46,14 @@ struct KPlan {
     uint8_t bl_nsrc[GP_MAX_BLOCKS];
     int8_t bl_src[GP_MAX_BLOCKS][GP_MAX_IEP_K];
     int8_t bl_slot[GP_MAX_BLOCKS];
+    // pair level (plain leaf): depths leaf-2 and leaf-1 evaluated together
+    // when the leaf set is f ∩ N(v) with f = g ∩ N(a) built at leaf-2, and
+    // the leaf-1 candidates and both bounds do not depend on a.  Then
+    //   sum_a sum_{v in C} |G(a) ∩ N(v) ∩ X| - sum_{a in C} |G(a) ∩ X|
+    // with G(a) = g ∩ N(a), X = [0, bound(leaf)) minus placed vertices.
+    int32_t pair;
+    int32_t pair_g;   // prefix g, -1 = every vertex
 };
 
 // Returns GP_OK or an error code (with gp_fail message) for unsupported input.
-int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, KPlan* out);
+int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool allow_pair, KPlan* out);

@@ -38,7 +38,7 @@ uint32_t restrict_mask(uint32_t m, int dp, const uint8_t* rmask) {
 
 } // namespace
 
-int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, KPlan* kp) {
+int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool allow_pair, KPlan* kp) {
     std::memset(kp, 0, sizeof(*kp));
     const int n = s->size;
     if (n < 2 || n > GP_MAX_SIZE) return gp_fail(GP_ERANGE, "pattern size must be 2..8");
@@ -218,6 +218,22 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, KPlan
             fill_block(0, 1);
         } else {
             for (int b = 0; b < kp->nblocks; ++b) fill_block(b, kp->block_mask[b]);
+        }
+    }
+
+    // ---- pair level ----
+    kp->pair = 0;
+    kp->pair_g = -1;
+    if (allow_pair && kp->batched && k == 0 && kp->leaf >= 4) {
+        const int L = kp->leaf;
+        const int q = kp->loop_set[L];
+        const int f = kp->father[q];
+        const int c = kp->loop_set[L - 1];
+        if (kp->depth_of[q] == L - 1 && f >= 0 && kp->depth_of[f] == L - 2 && kp->nbuild[L - 2] == 1 &&
+            kp->nbuild[L - 1] == 1 && kp->depth_of[c] < L - 2 && !(kp->rmask[L - 1] >> (L - 2) & 1) &&
+            kp->rmask[L] < (1u << (L - 2))) {
+            kp->pair = 1;
+            kp->pair_g = kp->father[f];
         }
     }
     return GP_OK;

@@ -91,7 +91,8 @@ typedef struct gp_tuning {
     int32_t no_truncate;    /* 1 = disable restriction-bound truncation */
     int32_t collect_balg;   /* 1 = run the reference-faithful loop nest and count B_alg */
     int32_t no_batch;       /* 1 = evaluate the last loop level one candidate at a time */
-    int32_t reserved[2];
+    int32_t no_pair;        /* 1 = never evaluate the last two loop levels together */
+    int32_t reserved[1];
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */
@@ -101,6 +102,9 @@ typedef struct gp_stats {
     uint64_t tasks;         /* depth-1 tasks processed */
     int32_t waves;          /* persistent waves launched */
     int32_t kernel_variant; /* which kernel ran */
+    double finish_p50_ms;   /* time after kernel start by which half the waves had finished */
+    double finish_p90_ms;   /* ... 90% of the waves */
+    double finish_max_ms;   /* ... every wave (load-balance tail) */
 } gp_stats;
 
 typedef struct gp_graph gp_graph; /* device-resident CSR (opaque) */

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 SETTINGS = [(1, 1, 1), (1, 1, 0), (0, 1, 0), (0, 0, 0), (0, 2, 0)]
 SMALL_PATTERNS = ["edge", "wedge", "triangle", "rectangle", "qg3", "house", "pentagon", "4clique", "5clique",
-                  "tri_edge_tri", "hourglass", "cycle_6_tri"]
+                  "tri_edge_tri", "hourglass", "cycle_6_tri", "6cycle", "5path", "4path"]
 
 
 def _tc(host):
@@ -134,6 +134,29 @@ def test_batched_level_matches_per_candidate(rmat13_path):
         print(f"{name} ({pm},{rt},{iep}) batched {a.stats.kernel_ms:.2f} ms per-candidate "
               f"{b.stats.kernel_ms:.2f} ms", flush=True)
         assert a.raw == b.raw, (name, pm, rt, iep)
+
+
+@pytest.mark.parametrize("name", ["pentagon", "6cycle", "5path", "4path", "house", "4clique"])
+def test_pair_level_matches(name):
+    host = gp.HostGraph.rmat(11, 8, seed=7)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    no_pair = gp.Tuning()
+    no_pair.no_pair = 1
+    no_batch = gp.Tuning()
+    no_batch.no_batch = 1
+    for pm, rt, iep in [(1, 1, 1), (1, 1, 0), (0, 1, 0)]:
+        s, valid = gp.build_schedule(gp.PATTERNS[name], pm, rt, iep, host.v_cnt, host.e_cnt, tri)
+        if not valid:
+            continue
+        a, b, c = dev.count(s), dev.count(s, no_pair), dev.count(s, no_batch)
+        print(f"{name} ({pm},{rt},{iep}) pair {a.stats.kernel_ms:.2f} ms batched {b.stats.kernel_ms:.2f} ms "
+              f"per-candidate {c.stats.kernel_ms:.2f} ms", flush=True)
+        assert a.raw == b.raw == c.raw, (name, pm, rt, iep)
+    small = _random_graph(150, 0.06, 9, hubs=4)
+    ds = gp.DeviceGraph(small)
+    s = _sched(small, gp.PATTERNS[name], 1, 1, 1)
+    assert ds.count(s).raw == ob.pattern_matching(small, s, threads=4)["raw"]
 
 
 def test_no_truncation_same_counts(rmat13_path):
