@@ -1,17 +1,23 @@
 """Benchmark: GraphPi pattern counting on one MI355X.
 
-Workload (default): R-MAT scale 22, edge factor 8, (a,b,c,d)=(0.57,0.19,0.19,0.05),
+Graph (default): R-MAT scale 22, edge factor 8, (a,b,c,d)=(0.57,0.19,0.19,0.05),
 seed 2, symmetrised and deduplicated (2,009,103 vertices, 32.6M undirected
-edges, int32 CSR); count 4-clique, 5-cycle and House with the planner's
-schedule (performance_modeling_type=1, restricts_type=1, IEP on).
+edges, int32 CSR), schedules from the planner (performance_modeling_type=1,
+restricts_type=1, IEP on).  This graph's configuration names 4-clique, 5-cycle
+and House; with the GraphPi schedules the 5-cycle and House loop nests take
+hours and tens of minutes on one MI355X, so the default step counts the
+4-clique and the others are selected with --patterns house,pentagon.
 
-One step = counting all three patterns over the resident CSR (one pass of the
-hot path).  The CSR is uploaded and the schedules are built before the timed
-region.  Prints one JSON line.
+One step = one full count of each selected pattern over the resident CSR,
+launched as consecutive task ranges (the raw sums add up exactly).  The CSR
+is uploaded and the schedules are built before the timed region.  Prints one
+JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2|cfg1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2|cfg1]
+                    [--patterns a,b] [--no-cpu] [--no-balg]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -24,16 +30,16 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     "cfg2": {
-        "workload": "R-MAT scale 22, edge factor 8, (0.57,0.19,0.19,0.05), seed 2; 4-clique, 5-cycle, House; "
-                    "pm=1 rt=1 IEP on",
+        "workload": "R-MAT scale 22, edge factor 8, (0.57,0.19,0.19,0.05), seed 2; pm=1 rt=1 IEP on",
         "gen": ("rmat", 22, 8, 2),
-        "patterns": ["4clique", "pentagon", "house"],
+        "patterns": ["4clique"],
+        "named": ["4clique", "pentagon", "house"],
     },
     "cfg1": {
-        "workload": "R-MAT scale 13, edge factor 12, (0.57,0.19,0.19,0.05), seed 1; Triangle, Rectangle, House; "
-                    "pm=1 rt=1 IEP on",
+        "workload": "R-MAT scale 13, edge factor 12, (0.57,0.19,0.19,0.05), seed 1; pm=1 rt=1 IEP on",
         "gen": ("rmat", 13, 12, 1),
         "patterns": ["triangle", "rectangle", "house"],
+        "named": ["triangle", "rectangle", "house"],
     },
 }
 
@@ -92,7 +98,19 @@ def main():
     ap.add_argument("--no-balg", action="store_true")
     ap.add_argument("--patterns", default=None, help="comma-separated subset (exploration only)")
     ap.add_argument("--scale", type=int, default=None, help="override the R-MAT scale (exploration only)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 4, "bytes_written": n * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (the counting path has no multi-GPU mode here); "
               f"got --gpus {args.gpus}", file=sys.stderr)
@@ -119,53 +137,127 @@ def main():
         assert valid
         scheds[name] = s
 
+    totals = {name: dev.count_tasks(scheds[name]) for name in cfg["patterns"]}
+
+    def count_pattern(name, target_ms=20000.0, tuning=None, quiet=False):
+        """One full count of `name`, launched as consecutive task ranges sized
+        so that each launch takes about target_ms (the first is 1/512 of the
+        tasks); the raw sums of the ranges add up to the whole count."""
+        s = scheds[name]
+        total = totals[name]
+        tun = gp.Tuning()
+        if tuning is not None:
+            ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
+        # small task spaces run as one launch
+        first = total if total <= (1 << 21) else max(1, total // 512)
+        begin, size, raw, ms, launches = 0, first, 0, 0.0, 0
+        balg_sum = 0
+        t0 = time.perf_counter()
+        while begin < total:
+            tun.task_begin = begin
+            tun.task_end = min(total, begin + size)
+            r = dev.count(s, tun)
+            raw = (raw + r.raw) % (1 << 64)
+            ms += r.stats.kernel_ms
+            balg_sum += r.stats.balg_bytes
+            launches += 1
+            done = tun.task_end
+            k = r.stats.kernel_ms
+            if not quiet:
+                log(f"    {name}: tasks {begin}..{done} of {total}: {k:.1f} ms (tail: half {r.stats.finish_p50_ms:.1f}, "
+                    f"all {r.stats.finish_max_ms:.1f} ms)")
+            grow = target_ms / max(k, 1.0)
+            size = max(1, int((done - begin) * min(4.0, max(0.25, grow))))
+            begin = done
+        signed = raw - (1 << 64) if raw >= (1 << 63) else raw
+        red = s.redundancy or 1
+        return {"count": signed // red if signed >= 0 else -((-signed) // red), "raw": raw, "kernel_ms": ms,
+                "launches": launches, "wall_s": time.perf_counter() - t0, "balg": balg_sum}
+
     def step():
         out = {}
         for name in cfg["patterns"]:
             log(f"  counting {name} ...")
-            out[name] = dev.count(scheds[name])
-            st = out[name].stats
-            log(f"  {name}: {out[name].count} in {st.kernel_ms:.2f} ms (waves done: half {st.finish_p50_ms:.1f} ms, "
-                f"90% {st.finish_p90_ms:.1f} ms, all {st.finish_max_ms:.1f} ms)")
+            out[name] = count_pattern(name)
+            log(f"  {name}: {out[name]['count']} in {out[name]['kernel_ms']:.1f} ms kernel time "
+                f"({out[name]['launches']} launches)")
         return out
 
     for _ in range(args.warmup):
-        r = step()
-        log("warmup " + " ".join(f"{n}={r[n].count} ({r[n].stats.kernel_ms:.1f} ms)" for n in r))
+        step()
     kernel_ms = {n: 0.0 for n in cfg["patterns"]}
     counts = {}
+    launches = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = step()
         for n in r:
-            kernel_ms[n] += r[n].stats.kernel_ms
-            counts[n] = r[n].count
+            kernel_ms[n] += r[n]["kernel_ms"]
+            counts[n] = r[n]["count"]
+            launches[n] = r[n]["launches"]
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
     for n in kernel_ms:
         kernel_ms[n] /= args.steps
     log("timed " + " ".join(f"{n}: {kernel_ms[n]:.2f} ms" for n in kernel_ms))
 
+    # B_alg: the reference-faithful kernel (no truncation, no batching) tallies
+    # 4 B x (|A|+|B|) per intersection + 8 B per neighbour-list lookup.  It
+    # covers the whole task space when the pattern's count takes under 10 s;
+    # otherwise it runs on 16 task ranges spread over the task space (1/16 of
+    # the tasks) and the total is scaled by the task fraction.
     per_pattern = {}
     balg = {}
+    balg_note = {}
     if not args.no_balg:
         tun = gp.Tuning()
         tun.collect_balg = 1
         for n in cfg["patterns"]:
-            rf = dev.count(scheds[n], tun)
-            assert rf.count == counts[n], (n, rf.count, counts[n])
-            balg[n] = rf.stats.balg_bytes
+            total = totals[n]
+            if kernel_ms[n] < 10000.0:
+                rf = count_pattern(n, tuning=tun, quiet=True)
+                assert rf["count"] == counts[n], (n, rf["count"], counts[n])
+                balg[n] = rf["balg"]
+                balg_note[n] = "exact"
+                log(f"  B_alg {n}: {balg[n]} bytes (faithful kernel {rf['kernel_ms']:.0f} ms)")
+            else:
+                parts, sampled, b = 16, 0, 0
+                width = total // (parts * 16)
+                for i in range(parts):
+                    tun.task_begin = total * i // parts
+                    tun.task_end = tun.task_begin + width
+                    rf = dev.count(scheds[n], tun)
+                    b += rf.stats.balg_bytes - (8 * host.v_cnt if i == 0 else 0)
+                    sampled += width
+                    log(f"    B_alg sample {n} {i + 1}/{parts}: {rf.stats.balg_bytes} bytes")
+                balg[n] = int(b * total / sampled) + 8 * host.v_cnt
+                balg_note[n] = f"estimated from {parts} task ranges covering {sampled} of {total} tasks"
+    traffic = {}
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            traffic = json.load(f)
+    except (OSError, ValueError):
+        traffic = {}
     for n in cfg["patterns"]:
-        per_pattern[n] = {"count": counts[n], "kernel_ms": round(kernel_ms[n], 3)}
+        per_pattern[n] = {"count": counts[n], "kernel_ms": round(kernel_ms[n], 3), "launches": launches[n]}
         if n in balg:
             per_pattern[n]["balg_bytes"] = balg[n]
             per_pattern[n]["balg_GBps"] = round(balg[n] / (kernel_ms[n] * 1e-3) / 1e9, 1)
+            per_pattern[n]["balg"] = balg_note[n]
     dom = max(kernel_ms, key=kernel_ms.get)
     roofline = None
     if dom in balg:
         achieved = balg[dom] / (kernel_ms[dom] * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"gp_count_kernel[{dom}]"}
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"gp_count_kernel[{dom}]",
+                    "achieved_basis": "reference loop-nest bytes (B_alg) / kernel time"}
+        key = f"{args.config}:{scale}:{dom}"
+        if key in traffic:
+            t = traffic[key]
+            roofline["traffic"] = t["hbm_bytes"]
+            roofline["traffic_GBps"] = round(t["hbm_bytes"] / (kernel_ms[dom] * 1e-3) / 1e9, 1)
+            roofline["traffic_frac"] = round(roofline["traffic_GBps"] / HBM_PEAK_GBS, 4)
+            roofline["traffic_source"] = t["source"]
 
     cpu = None
     if not args.no_cpu:
@@ -187,7 +279,7 @@ def main():
         "dtype": "int64",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "v_cnt": host.v_cnt, "undirected_edges": host.e_cnt // 2,
-                   "triangles": tri, "patterns": cfg["patterns"]},
+                   "triangles": tri, "patterns": cfg["patterns"], "patterns_named": cfg["named"]},
         "per_pattern": per_pattern,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -95,7 +95,9 @@ class Tuning(ctypes.Structure):
         ("collect_balg", ctypes.c_int32),
         ("no_batch", ctypes.c_int32),
         ("no_pair", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("reserved", ctypes.c_int32),
+        ("task_begin", ctypes.c_int64),
+        ("task_end", ctypes.c_int64),
     ]
 
 
@@ -135,6 +137,7 @@ _SIGS = {
     "gp_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(Tuning),
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(Stats)]),
+    "gp_count_tasks": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(ctypes.c_uint64)]),
     "gp_tc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]),
     "gp_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
     "gp_last_error": (ctypes.c_char_p, []),
@@ -374,6 +377,40 @@ class DeviceGraph:
         _check(lib().gp_count(self._h, ctypes.byref(sched), ctypes.byref(tuning) if tuning else None,
                               ctypes.byref(raw), ctypes.byref(cnt), ctypes.byref(st)))
         return CountResult(cnt.value, raw.value, st)
+
+    def count_tasks(self, sched):
+        t = ctypes.c_uint64()
+        _check(lib().gp_count_tasks(self._h, ctypes.byref(sched), ctypes.byref(t)))
+        return t.value
+
+    def count_sliced(self, sched, slices, tuning=None, progress=None):
+        """The same count as count(), run as `slices` launches over disjoint
+        task ranges (raw sums add up); progress(i, result) after each."""
+        total = self.count_tasks(sched)
+        raw = 0
+        ms = 0.0
+        balg = 0
+        t = Tuning()
+        if tuning is not None:
+            ctypes.memmove(ctypes.byref(t), ctypes.byref(tuning), ctypes.sizeof(Tuning))
+        slices = max(1, min(slices, total if total else 1))
+        for i in range(slices):
+            t.task_begin = total * i // slices
+            t.task_end = total * (i + 1) // slices
+            r = self.count(sched, t)
+            raw = (raw + r.raw) % (1 << 64)
+            ms += r.stats.kernel_ms
+            balg += r.stats.balg_bytes
+            if progress:
+                progress(i, r)
+        signed = raw - (1 << 64) if raw >= (1 << 63) else raw
+        red = sched.redundancy or 1
+        st = Stats()
+        st.kernel_ms = ms
+        st.balg_bytes = balg
+        q = abs(signed) // abs(red)
+        count = q if (signed >= 0) == (red > 0) else -q
+        return CountResult(count, raw, st)
 
     def pattern_matching(self, sched):
         """Graph::pattern_matching: the count divided by the IEP redundancy."""

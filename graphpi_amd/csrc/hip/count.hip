@@ -53,6 +53,7 @@ struct KArgs {
     int32_t v_cnt;
     int32_t chunk;
     uint64_t n_tasks;
+    uint64_t task_base;
     unsigned long long* counter;
     int32_t* scratch;
     int64_t slot_stride;
@@ -755,7 +756,8 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         base = __shfl(base, 0);
         if (base >= a.n_tasks) break;
         const uint64_t end = min((uint64_t)base + (uint64_t)a.chunk, a.n_tasks);
-        for (uint64_t t = base; t < end; ++t) {
+        for (uint64_t tt = base; tt < end; ++tt) {
+            const uint64_t t = a.task_base + tt;
             ++taken;
             int32_t v0, v1 = -1;
             if (P.root_mode) {
@@ -977,7 +979,15 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 16;
     int chunk = tuning && tuning->chunk > 0 ? tuning->chunk : 16;
     int64_t waves = (int64_t)g->num_cu * wpc;
-    const uint64_t n_tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+    const uint64_t all_tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+    uint64_t t_begin = 0, t_end = all_tasks;
+    if (tuning) {
+        if (tuning->task_begin < 0 || (tuning->task_end > 0 && tuning->task_end < tuning->task_begin))
+            return gp_fail(GP_EINVAL, "bad task range");
+        t_begin = std::min<uint64_t>((uint64_t)tuning->task_begin, all_tasks);
+        if (tuning->task_end > 0) t_end = std::min<uint64_t>((uint64_t)tuning->task_end, all_tasks);
+    }
+    const uint64_t n_tasks = t_end > t_begin ? t_end - t_begin : 0;
     if (waves < 1) waves = 1;
     if ((uint64_t)waves > n_tasks / (uint64_t)chunk + 1) waves = (int64_t)(n_tasks / (uint64_t)chunk + 1);
     // scratch: per wave, one slot per materialised prefix (a subset of one
@@ -1014,6 +1024,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     a.v_cnt = g->v_cnt;
     a.chunk = chunk;
     a.n_tasks = n_tasks;
+    a.task_base = t_begin;
     a.counter = c->d_cnt;
     a.scratch = c->d_scratch;
     a.slot_stride = slot_stride;
@@ -1056,8 +1067,11 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     if (count) *count = (int64_t)sum / (sched->redundancy ? sched->redundancy : 1);
     if (stats) {
         stats->kernel_ms = ms;
-        // every root's neighbour list is looked up once by the reference
-        stats->balg_bytes = faithful ? host_out[2] + 8ull * (uint64_t)g->v_cnt : 0;
+        // every root's neighbour list is looked up once by the reference;
+        // those lookups are attributed to the range that starts at task 0
+        stats->balg_bytes = faithful ? host_out[2] + (t_begin == 0 && !kp.root_mode ? 8ull * (uint64_t)g->v_cnt : 0) +
+                                           (kp.root_mode ? 8ull * n_tasks : 0)
+                                     : 0;
         stats->tasks = host_out[3];
         stats->waves = (int32_t)waves;
         stats->kernel_variant = faithful ? 1 : 0;
@@ -1151,6 +1165,15 @@ extern "C" int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_
                         int64_t* count, gp_stats* stats) {
     if (!g || !sched) return gp_fail(GP_EINVAL, "null argument");
     return run_count(const_cast<gp_graph*>(g), sched, tuning, raw_sum, count, stats);
+}
+
+extern "C" int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks) {
+    if (!g || !sched || !tasks) return gp_fail(GP_EINVAL, "null argument");
+    KPlan kp;
+    int rc = gp_make_kplan(sched, false, true, true, &kp);
+    if (rc != GP_OK) return rc;
+    *tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+    return GP_OK;
 }
 
 extern "C" int gp_tc(const gp_graph* g, uint64_t* triangles, gp_stats* stats) {

@@ -92,7 +92,13 @@ typedef struct gp_tuning {
     int32_t collect_balg;   /* 1 = run the reference-faithful loop nest and count B_alg */
     int32_t no_batch;       /* 1 = evaluate the last loop level one candidate at a time */
     int32_t no_pair;        /* 1 = never evaluate the last two loop levels together */
-    int32_t reserved[1];
+    int32_t reserved;
+    /* Count only tasks [task_begin, task_end) (task_end <= 0: to the end).
+     * Tasks are directed CSR edge slots, or root vertices when the pattern's
+     * leaf is at depth 1 (gp_count_tasks gives the total).  Raw sums of
+     * disjoint ranges add up to the raw sum of the whole count. */
+    int64_t task_begin;
+    int64_t task_end;
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */
@@ -127,6 +133,10 @@ GP_API int gp_graph_bytes(const gp_graph* g, uint64_t* bytes);
  * thread; uses its own HIP stream. */
 GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
                     uint64_t* raw_sum, int64_t* count, gp_stats* stats);
+
+/* Number of tasks gp_count splits the schedule's count into (see
+ * gp_tuning.task_begin). */
+GP_API int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks);
 
 /* Triangle count with each triangle counted once (Graph::triangle_counting_mt,
  * src/graph.cpp:112-139). */

@@ -159,6 +159,27 @@ def test_pair_level_matches(name):
     assert ds.count(s).raw == ob.pattern_matching(small, s, threads=4)["raw"]
 
 
+def test_task_ranges_add_up(rmat13_path):
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    for name in ["house", "pentagon", "4clique", "wedge", "triangle"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        whole = dev.count(s)
+        for slices in (1, 7, 64):
+            part = dev.count_sliced(s, slices)
+            assert (part.raw, part.count) == (whole.raw, whole.count), (name, slices)
+    # faithful B_alg is additive over ranges too
+    s = _sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)
+    tun = gp.Tuning()
+    tun.collect_balg = 1
+    assert dev.count_sliced(s, 5, tun).stats.balg_bytes == dev.count(s, tun).stats.balg_bytes
+    with pytest.raises(gp.GPError):
+        bad = gp.Tuning()
+        bad.task_begin, bad.task_end = 10, 5
+        dev.count(s, bad)
+
+
 def test_no_truncation_same_counts(rmat13_path):
     host = gp.HostGraph.load_text(rmat13_path)
     dev = gp.DeviceGraph(host)
