@@ -37,11 +37,15 @@
 #include "schedule.h"
 
 #define WAVE 64
-#define GP_HASH_SLOTS 2048      // LDS hash entries per wave for the batched leaf level
 #define GP_MAX_BATCH_BLOCKS 15  // leaf blocks the batched level handles
-#define GP_PAIR_LG 10           // pair level: count map of 1024 keys + 1024 counts (the hash pool)
-#define GP_PAIR_SLOTS (1 << GP_PAIR_LG)
-static_assert(2 * GP_PAIR_SLOTS <= GP_HASH_SLOTS, "pair map must fit the LDS hash pool");
+#define GP_LONG_LIST 64         // candidate lists at least this long are streamed by the whole wave
+#define GP_UNROLL 4             // independent loads in flight per lane on a long list
+// LDS hash pool per wave (dynamic shared memory, a power of two): the staged
+// leaf sets of the batched level, or the pair level's count map (keys in the
+// first half, counts in the second)
+#define GP_HASH_SLOTS_BATCH 1024
+#define GP_HASH_SLOTS_PAIR 2048
+#define GP_HASH_SLOTS_MAX 8192
 
 namespace {
 
@@ -60,6 +64,8 @@ struct KArgs {
     int64_t wave_stride;
     int64_t tmp_stride;
     int64_t bslot_stride;
+    int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
+    int32_t hash_lg;
     unsigned long long* out; // [0] raw sum, [1] B_alg bytes, [2] tasks taken
     unsigned long long* clocks; // per wave: start, finish (wall clock ticks)
 };
@@ -229,13 +235,15 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     __shared__ int32_t s_li[GP_MAX_SIZE];
     __shared__ int32_t s_blk[GP_MAX_BLOCKS];
     // batched leaf level
-    __shared__ int32_t s_hash[GP_HASH_SLOTS];
-    __shared__ int32_t s_bc[GP_MAX_BATCH_BLOCKS][WAVE];
+    // dynamic: the hash pool, then one row of 64 per-candidate counts per leaf block
+    extern __shared__ int32_t s_dyn[];
+    int32_t* const s_hash = s_dyn;
+    int32_t(*const s_bc)[WAVE] = reinterpret_cast<int32_t(*)[WAVE]>(s_dyn + a.hash_slots);
     __shared__ const int32_t* s_bg[GP_MAX_BATCH_BLOCKS];
     __shared__ int32_t s_bn[GP_MAX_BATCH_BLOCKS];
     __shared__ int32_t s_bh[GP_MAX_BATCH_BLOCKS];  // LDS hash offset, -1 = not staged
     __shared__ int32_t s_blg[GP_MAX_BATCH_BLOCKS];
-    __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kind 0)
+    __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kinds 0 and 1)
     __shared__ int32_t s_off[WAVE + 1];
     __shared__ int32_t s_off2[WAVE + 1];
 
@@ -442,15 +450,16 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             s_bn[b] = gn;
             s_bh[b] = -1;
             s_blg[b] = 0;
-            if (kind == 0) {
+            if (kind != 2) {
                 bool hit = lane < d && gn > 0 && contains(g, gn, s_emb[lane]);
                 s_bemb[b] = (int32_t)__ballot(hit);
-            } else if (kind == 1) {
+            }
+            if (kind == 1) {
                 ++nstream;
                 stream_total += gn;
                 if (gn > 0) smax = max(smax, g[gn - 1]);
                 const int lg = max(6, ilog2_ceil(2 * gn));
-                if (gn > 0 && hoff + (1 << lg) <= GP_HASH_SLOTS) {
+                if (gn > 0 && hoff + (1 << lg) <= a.hash_slots) {
                     hash_fill(s_hash + hoff, lg, g, gn);
                     s_bh[b] = hoff;
                     s_blg[b] = lg;
@@ -495,6 +504,10 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 probe = cut > 64 && probe_cost < stream_cost;
                 work = probe ? 0 : cut;
             }
+            // long lists go through the whole-wave path below
+            const bool longc = work >= GP_LONG_LIST && deg > 16;
+            const int32_t lwork = work;
+            if (longc) work = 0;
             for (int b2 = 0; b2 < nb; ++b2)
                 if (P.bl_kind[b2] == 1) s_bc[b2][lane] = 0;
             int32_t total = 0;
@@ -539,6 +552,44 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 }
             }
             __syncthreads();
+            // long lists: the whole wave streams one list, GP_UNROLL loads in
+            // flight per lane, and counts hits block by block (the list stays
+            // in cache for the next block).  Every element is below the cut,
+            // and embedded vertices in G are subtracted once per candidate.
+            uint64_t lm = __ballot(longc);
+            while (lm) {
+                const int j = __ffsll((long long)lm) - 1;
+                lm &= lm - 1;
+                const int32_t* pv = a.edge + __shfl(nvs, j);
+                const int32_t pn = __shfl(lwork, j);
+                for (int b2 = 0; b2 < nb; ++b2) {
+                    if (P.bl_kind[b2] != 1) continue;
+                    const int h = s_bh[b2];
+                    const int lg = s_blg[b2];
+                    const int32_t* g = s_bg[b2];
+                    const int32_t gn = s_bn[b2];
+                    int32_t c = 0;
+                    // wave-uniform trip count: every lane takes part in every ballot
+                    for (int32_t f0 = 0; f0 < pn; f0 += GP_UNROLL * WAVE) {
+                        const int32_t f = f0 + lane;
+                        int32_t x[GP_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < GP_UNROLL; ++u) x[u] = f + u * WAVE < pn ? pv[f + u * WAVE] : -1;
+#pragma unroll
+                        for (int u = 0; u < GP_UNROLL; ++u) {
+                            bool hit = false;
+                            if (x[u] >= 0) hit = h >= 0 ? hash_has(s_hash + h, lg, x[u]) : contains(g, gn, x[u]);
+                            c += popc64(__ballot(hit));
+                        }
+                    }
+                    const uint32_t em = (uint32_t)s_bemb[b2];
+                    if (em) {
+                        bool sub = lane < d && (em >> lane & 1) && contains(pv, pn, s_emb[lane]);
+                        c -= popc64(__ballot(sub));
+                    }
+                    if (lane == j) s_bc[b2][j] = c;
+                }
+            }
             // candidates whose lists are probed
             uint64_t pm = __ballot(probe);
             while (pm) {
@@ -611,9 +662,11 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         const int gpf = P.pair_g;
         const int32_t* gptr = gpf >= 0 ? s_ptr[gpf] : nullptr;
         const int32_t glen = gpf >= 0 ? s_len[gpf] : 0;
+        const int32_t pair_slots = a.hash_slots >> 1;
+        const int pair_lg = a.hash_lg - 1;
         int32_t* keys = s_hash;
-        int32_t* cnts = s_hash + GP_PAIR_SLOTS;
-        const int32_t cap_elems = GP_PAIR_SLOTS / 2;
+        int32_t* cnts = s_hash + pair_slots;
+        const int32_t cap_elems = pair_slots / 2;
         for (int32_t abase = 0; abase < Dn; abase += WAVE) {
             const int32_t ai = abase + lane;
             bool va = ai < Dn;
@@ -641,7 +694,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 if (end <= start) end = start + 1;
                 const int32_t batch_elems = __shfl(ex + sz, end - 1) - base_e;
                 if (batch_elems > 0) {
-                    for (int i = lane; i < GP_PAIR_SLOTS; i += WAVE) {
+                    for (int i = lane; i < pair_slots; i += WAVE) {
                         keys[i] = -1;
                         cnts[i] = 0;
                     }
@@ -661,14 +714,14 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                         for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
                         if (keep && gpf >= 0) keep = contains(gptr, glen, x);
                         if (keep) {
-                            uint32_t h = hash_slot(x, GP_PAIR_LG);
+                            uint32_t h = hash_slot(x, pair_lg);
                             for (;;) {
                                 const int32_t k = atomicCAS(&keys[h], -1, x);
                                 if (k == -1 || k == x) {
                                     atomicAdd(&cnts[h], 1);
                                     break;
                                 }
-                                h = (h + 1) & (GP_PAIR_SLOTS - 1);
+                                h = (h + 1) & (pair_slots - 1);
                             }
                         }
                     }
@@ -702,7 +755,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                             if (act) {
                                 const int32_t x = a.edge[os + (f - oe)];
                                 if (x < b) {
-                                    uint32_t h = hash_slot(x, GP_PAIR_LG);
+                                    uint32_t h = hash_slot(x, pair_lg);
                                     for (;;) {
                                         const int32_t k = keys[h];
                                         if (k == x) {
@@ -710,7 +763,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                                             break;
                                         }
                                         if (k == -1) break;
-                                        h = (h + 1) & (GP_PAIR_SLOTS - 1);
+                                        h = (h + 1) & (pair_slots - 1);
                                     }
                                 }
                             }
@@ -976,8 +1029,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
         std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
     }
     HIP_TRY(hipSetDevice(g->device));
-    int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 16;
-    int chunk = tuning && tuning->chunk > 0 ? tuning->chunk : 16;
+    int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 20;
     int64_t waves = (int64_t)g->num_cu * wpc;
     const uint64_t all_tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
     uint64_t t_begin = 0, t_end = all_tasks;
@@ -988,6 +1040,10 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
         if (tuning->task_end > 0) t_end = std::min<uint64_t>((uint64_t)tuning->task_end, all_tasks);
     }
     const uint64_t n_tasks = t_end > t_begin ? t_end - t_begin : 0;
+    // tasks per work fetch: 16, fewer when there are under 32 chunks per wave
+    // (small graphs: the last chunks then decide the kernel's length)
+    int chunk = tuning && tuning->chunk > 0 ? tuning->chunk
+                                            : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, n_tasks / (uint64_t)(waves * 32)));
     if (waves < 1) waves = 1;
     if ((uint64_t)waves > n_tasks / (uint64_t)chunk + 1) waves = (int64_t)(n_tasks / (uint64_t)chunk + 1);
     // scratch: per wave, one slot per materialised prefix (a subset of one
@@ -1001,6 +1057,15 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     const int64_t bslot_stride = kp.batched && kp.nbslots ? deg_stride : 0;
     const int64_t wave_stride = kp.nslots * slot_stride + 2 * tmp_stride + (kp.nbslots + 1) * bslot_stride;
     const size_t scratch_bytes = (size_t)std::max<int64_t>(waves * wave_stride, 64) * sizeof(int32_t);
+    // LDS hash pool: tuning->lds_stage elements staged at load factor <= 1/2
+    int hash_slots = kp.pair ? GP_HASH_SLOTS_PAIR : kp.batched ? GP_HASH_SLOTS_BATCH : 0;
+    if (tuning && tuning->lds_stage > 0 && (kp.pair || kp.batched)) {
+        hash_slots = 64;
+        while (hash_slots < 2 * tuning->lds_stage && hash_slots < GP_HASH_SLOTS_MAX) hash_slots <<= 1;
+    }
+    int hash_lg = 0;
+    while ((1 << hash_lg) < hash_slots) ++hash_lg;
+    const size_t lds_bytes = sizeof(int32_t) * ((size_t)hash_slots + (kp.batched ? (size_t)kp.nblocks * WAVE : 0));
 
     CallCtx* c = nullptr;
     rc = acquire_ctx(g, scratch_bytes, &c);
@@ -1031,6 +1096,8 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     a.wave_stride = wave_stride;
     a.tmp_stride = tmp_stride;
     a.bslot_stride = bslot_stride;
+    a.hash_slots = hash_slots;
+    a.hash_lg = hash_lg;
     a.out = c->d_cnt + 1;
     a.clocks = c->d_clocks;
     unsigned long long host_out[4] = {0, 0, 0, 0};
@@ -1043,9 +1110,9 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
     if (e == hipSuccess && n_tasks > 0) {
         if (faithful)
-            hipLaunchKernelGGL(gp_count_kernel<true>, dim3((unsigned)waves), dim3(WAVE), 0, c->stream, a);
+            hipLaunchKernelGGL(gp_count_kernel<true>, dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
         else
-            hipLaunchKernelGGL(gp_count_kernel<false>, dim3((unsigned)waves), dim3(WAVE), 0, c->stream, a);
+            hipLaunchKernelGGL(gp_count_kernel<false>, dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);

@@ -85,9 +85,12 @@ typedef struct gp_sched_flat {
 
 /* Counting options.  Zero-initialise and set what you need. */
 typedef struct gp_tuning {
-    int32_t waves_per_cu;   /* persistent waves per CU; 0 = default */
-    int32_t chunk;          /* depth-1 edge tasks per work fetch; 0 = default */
-    int32_t lds_stage;      /* max father-set elements staged in LDS; 0 = default */
+    int32_t waves_per_cu;   /* persistent waves per CU; 0 = default (20) */
+    int32_t chunk;          /* tasks per work fetch; 0 = 16, or fewer when there are under
+                               32 fetches per wave */
+    int32_t lds_stage;      /* set elements the LDS hash pool holds (pool = the power of two
+                               >= 2 x lds_stage, at most 8192 entries); 0 = default
+                               (512 for the batched leaf level, 1024 for the pair level) */
     int32_t no_truncate;    /* 1 = disable restriction-bound truncation */
     int32_t collect_balg;   /* 1 = run the reference-faithful loop nest and count B_alg */
     int32_t no_batch;       /* 1 = evaluate the last loop level one candidate at a time */
