@@ -72,10 +72,20 @@ struct KArgs {
 
 __device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
 
+// Every set pointer in this kernel points into device memory (the CSR or the
+// per-wave scratch), but many pass through LDS on their way, which makes the
+// compiler fall back to flat addressing; reading through these types gives
+// plain global loads that do not wait on LDS traffic.
+typedef const int32_t __attribute__((address_space(1)))* gcptr;
+typedef int32_t __attribute__((address_space(1)))* gptr;
+#define GL(p) ((gcptr)(p))
+#define GLW(p) ((gptr)(p))
+
 __device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
 
 // sorted-list membership, one lane one probe sequence
-__device__ __forceinline__ bool contains(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+__device__ __forceinline__ bool contains(const int32_t* __restrict__ ap, int32_t n, int32_t x) {
+    const gcptr a = GL(ap);
     int32_t lo = 0, len = n;
     while (len > 0) {
         int32_t half = len >> 1;
@@ -88,7 +98,8 @@ __device__ __forceinline__ bool contains(const int32_t* __restrict__ a, int32_t 
 }
 
 // first index with a[i] >= x; wave-uniform result
-__device__ int32_t wave_lower_bound(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+__device__ int32_t wave_lower_bound(const int32_t* __restrict__ ap, int32_t n, int32_t x) {
+    const gcptr a = GL(ap);
     const int lane = lane_id();
     int32_t lo = 0, hi = n;
     while (hi - lo > WAVE) {
@@ -119,9 +130,10 @@ __device__ int32_t count_emb_in(const int32_t* __restrict__ a, int32_t n, const 
 // emb[0:m).  Otherwise the sorted result is written to out (no exclusion).
 template <bool COUNT>
 __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const int32_t* __restrict__ B,
-                             int32_t nb, int32_t ub, int32_t* __restrict__ out, const int32_t* emb,
+                             int32_t nb, int32_t ub, int32_t* __restrict__ outp, const int32_t* emb,
                              int m) {
     const int lane = lane_id();
+    const gptr out = GLW(outp);
     const int32_t* S = A;
     const int32_t* Lg = B;
     int32_t ns = na, nl = nb;
@@ -136,7 +148,7 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
     for (int32_t base = 0; base < ns; base += WAVE) {
         int32_t i = base + lane;
         bool in = i < ns;
-        int32_t x = in ? S[i] : 0;
+        int32_t x = in ? GL(S)[i] : 0;
         bool keep = in && x < ub;
         bool hit = keep && contains(Lg, nl, x);
         if (COUNT)
@@ -175,7 +187,8 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
 }
 
 // lane-private lower bound (first index with a[i] >= x)
-__device__ __forceinline__ int32_t lower_bound_lane(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+__device__ __forceinline__ int32_t lower_bound_lane(const int32_t* __restrict__ ap, int32_t n, int32_t x) {
+    const gcptr a = GL(ap);
     int32_t lo = 0, len = n;
     while (len > 0) {
         int32_t half = len >> 1;
@@ -224,6 +237,36 @@ __device__ __forceinline__ bool hash_has(const int32_t* tab, int lg, int32_t x) 
 }
 
 __device__ __forceinline__ int ilog2_ceil(int32_t x) { return x <= 1 ? 0 : 32 - __clz(x - 1); }
+
+// A sorted set copied into LDS and padded with INT_MAX to 2^lg entries
+// (2^lg > n).  Membership is a branch-free binary search of exactly lg steps,
+// the same for every lane, so a wave never diverges on it.
+__device__ void stage_sorted(int32_t* tab, int lg, const int32_t* __restrict__ g, int32_t n) {
+    const gcptr src = GL(g);
+    for (int32_t i = lane_id(); i < (1 << lg); i += WAVE) tab[i] = i < n ? src[i] : INT_MAX;
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool staged_has(const int32_t* tab, int lg, int32_t x) {
+    int32_t pos = 0;
+    for (int32_t step = (1 << lg) >> 1; step > 0; step >>= 1) pos = tab[pos + step - 1] < x ? pos + step : pos;
+    return tab[pos] == x;
+}
+
+// GP_UNROLL searches interleaved step by step, so their LDS reads overlap
+__device__ __forceinline__ uint32_t staged_has_n(const int32_t* tab, int lg, const int32_t* x) {
+    int32_t pos[GP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) pos[u] = 0;
+    for (int32_t step = (1 << lg) >> 1; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < GP_UNROLL; ++u) pos[u] = tab[pos[u] + step - 1] < x[u] ? pos[u] + step : pos[u];
+    }
+    uint32_t hits = 0;
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) hits |= (tab[pos[u]] == x[u] ? 1u : 0u) << u;
+    return hits;
+}
 
 template <bool FAITHFUL>
 __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
@@ -308,7 +351,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                     best = q;
                 }
             }
-        const int32_t* S = s_ptr[best];
+        const gcptr S = GL(s_ptr[best]);
         int32_t total = 0;
         for (int32_t base = 0; base < best_len; base += WAVE) {
             int32_t i = base + lane;
@@ -421,7 +464,10 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         // 1. the fixed set G of every block
         int32_t smax = -1;       // largest element of any streamed set
         int nstream = 0;
-        int stream_cost_w = 0;   // per streamed element, LDS or global lookups
+        // stream or probe: per streamed element, the lookup steps into every
+        // block's set (LDS steps count a quarter); per probed set element, one
+        // binary search of the list at twice the weight
+        int stream_cost_w = 0;
         int32_t stream_total = 0;
         int hoff = 0;
         for (int b = 0; b < nb; ++b) {
@@ -457,14 +503,14 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             if (kind == 1) {
                 ++nstream;
                 stream_total += gn;
-                if (gn > 0) smax = max(smax, g[gn - 1]);
-                const int lg = max(6, ilog2_ceil(2 * gn));
+                if (gn > 0) smax = max(smax, GL(g)[gn - 1]);
+                const int lg = ilog2_ceil(gn + 1);
                 if (gn > 0 && hoff + (1 << lg) <= a.hash_slots) {
-                    hash_fill(s_hash + hoff, lg, g, gn);
+                    stage_sorted(s_hash + hoff, lg, g, gn);
                     s_bh[b] = hoff;
                     s_blg[b] = lg;
                     hoff += 1 << lg;
-                    stream_cost_w += 1;
+                    stream_cost_w += 1 + lg / 4;
                 } else {
                     stream_cost_w += ilog2_ceil(gn + 1) + 1;
                 }
@@ -542,7 +588,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                     bool hit = false;
                     if (keep) {
                         const int h = s_bh[b2];
-                        hit = h >= 0 ? hash_has(s_hash + h, s_blg[b2], x) : contains(s_bg[b2], s_bn[b2], x);
+                        hit = h >= 0 ? staged_has(s_hash + h, s_blg[b2], x) : contains(s_bg[b2], s_bn[b2], x);
                     }
                     const uint64_t hm = __ballot(hit);
                     if (start) {
@@ -560,7 +606,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             while (lm) {
                 const int j = __ffsll((long long)lm) - 1;
                 lm &= lm - 1;
-                const int32_t* pv = a.edge + __shfl(nvs, j);
+                const gcptr pv = GL(a.edge) + __shfl(nvs, j);
                 const int32_t pn = __shfl(lwork, j);
                 for (int b2 = 0; b2 < nb; ++b2) {
                     if (P.bl_kind[b2] != 1) continue;
@@ -575,16 +621,19 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                         int32_t x[GP_UNROLL];
 #pragma unroll
                         for (int u = 0; u < GP_UNROLL; ++u) x[u] = f + u * WAVE < pn ? pv[f + u * WAVE] : -1;
+                        if (h >= 0) {
+                            // x = -1 (past the end) never matches a staged entry
+                            const uint32_t hits = staged_has_n(s_hash + h, lg, x);
 #pragma unroll
-                        for (int u = 0; u < GP_UNROLL; ++u) {
-                            bool hit = false;
-                            if (x[u] >= 0) hit = h >= 0 ? hash_has(s_hash + h, lg, x[u]) : contains(g, gn, x[u]);
-                            c += popc64(__ballot(hit));
+                            for (int u = 0; u < GP_UNROLL; ++u) c += popc64(__ballot(hits >> u & 1));
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < GP_UNROLL; ++u) c += popc64(__ballot(x[u] >= 0 && contains(g, gn, x[u])));
                         }
                     }
                     const uint32_t em = (uint32_t)s_bemb[b2];
                     if (em) {
-                        bool sub = lane < d && (em >> lane & 1) && contains(pv, pn, s_emb[lane]);
+                        bool sub = lane < d && (em >> lane & 1) && contains((const int32_t*)pv, pn, s_emb[lane]);
                         c -= popc64(__ballot(sub));
                     }
                     if (lane == j) s_bc[b2][j] = c;
