@@ -140,16 +140,17 @@ def main():
     totals = {name: dev.count_tasks(scheds[name]) for name in cfg["patterns"]}
 
     def count_pattern(name, target_ms=20000.0, tuning=None, quiet=False):
-        """One full count of `name`, launched as consecutive task ranges sized
-        so that each launch takes about target_ms (the first is 1/512 of the
-        tasks); the raw sums of the ranges add up to the whole count."""
+        """One full count of `name` as consecutive task ranges: a first range
+        of 1/64 of the tasks; then the rest in one launch when that is
+        predicted to take under a minute, else in ranges sized for about
+        target_ms each (progress is logged per range).  The raw sums of the
+        ranges add up to the whole count."""
         s = scheds[name]
         total = totals[name]
         tun = gp.Tuning()
         if tuning is not None:
             ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
-        # small task spaces run as one launch
-        first = total if total <= (1 << 21) else max(1, total // 512)
+        first = total if total <= (1 << 20) else max(1, total // 64)
         begin, size, raw, ms, launches = 0, first, 0, 0.0, 0
         balg_sum = 0
         t0 = time.perf_counter()
@@ -165,9 +166,12 @@ def main():
             k = r.stats.kernel_ms
             if not quiet:
                 log(f"    {name}: tasks {begin}..{done} of {total}: {k:.1f} ms (tail: half {r.stats.finish_p50_ms:.1f}, "
-                    f"all {r.stats.finish_max_ms:.1f} ms)")
-            grow = target_ms / max(k, 1.0)
-            size = max(1, int((done - begin) * min(4.0, max(0.25, grow))))
+                    f"all {r.stats.finish_max_ms:.1f} ms; {r.stats.split_tasks} tasks split)")
+            rate = max(k, 0.01) / max(1, done - begin)   # ms per task so far
+            if launches == 1 and rate * (total - done) < 60000.0:
+                size = total - done
+            else:
+                size = max(1, int(target_ms / rate))
             begin = done
         signed = raw - (1 << 64) if raw >= (1 << 63) else raw
         red = s.redundancy or 1
@@ -203,9 +207,10 @@ def main():
 
     # B_alg: the reference-faithful kernel (no truncation, no batching) tallies
     # 4 B x (|A|+|B|) per intersection + 8 B per neighbour-list lookup.  It
-    # covers the whole task space when the pattern's count takes under 10 s;
-    # otherwise it runs on 16 task ranges spread over the task space (1/16 of
-    # the tasks) and the total is scaled by the task fraction.
+    # first runs on 16 task ranges spread over the task space (1/256 of the
+    # tasks); when the whole faithful count is then predicted to take under
+    # 30 s it runs in full (exact), otherwise the sample is scaled by the task
+    # fraction (estimated).
     per_pattern = {}
     balg = {}
     balg_note = {}
@@ -214,23 +219,26 @@ def main():
         tun.collect_balg = 1
         for n in cfg["patterns"]:
             total = totals[n]
-            if kernel_ms[n] < 10000.0:
+            sz, k = scheds[n].size, scheds[n].iep_num
+            root_mode = (sz - k if k > 1 else sz - 1) == 1   # tasks are roots, not edge slots
+            parts, sampled, b, t_ms = 16, 0, 0, 0.0
+            width = max(1, total // (parts * 16))
+            for i in range(parts):
+                tun.task_begin = total * i // parts
+                tun.task_end = min(total, tun.task_begin + width)
+                rf = dev.count(scheds[n], tun)
+                b += rf.stats.balg_bytes - (8 * host.v_cnt if i == 0 and not root_mode else 0)
+                sampled += tun.task_end - tun.task_begin
+                t_ms += rf.stats.kernel_ms
+            log(f"  B_alg sample {n}: {b} bytes over {sampled} of {total} tasks in {t_ms:.0f} ms")
+            if t_ms * total / max(1, sampled) < 30000.0:
                 rf = count_pattern(n, tuning=tun, quiet=True)
                 assert rf["count"] == counts[n], (n, rf["count"], counts[n])
                 balg[n] = rf["balg"]
                 balg_note[n] = "exact"
                 log(f"  B_alg {n}: {balg[n]} bytes (faithful kernel {rf['kernel_ms']:.0f} ms)")
             else:
-                parts, sampled, b = 16, 0, 0
-                width = total // (parts * 16)
-                for i in range(parts):
-                    tun.task_begin = total * i // parts
-                    tun.task_end = tun.task_begin + width
-                    rf = dev.count(scheds[n], tun)
-                    b += rf.stats.balg_bytes - (8 * host.v_cnt if i == 0 else 0)
-                    sampled += width
-                    log(f"    B_alg sample {n} {i + 1}/{parts}: {rf.stats.balg_bytes} bytes")
-                balg[n] = int(b * total / sampled) + 8 * host.v_cnt
+                balg[n] = int(b * total / sampled) + (0 if root_mode else 8 * host.v_cnt)
                 balg_note[n] = f"estimated from {parts} task ranges covering {sampled} of {total} tasks"
     traffic = {}
     try:

@@ -95,7 +95,7 @@ class Tuning(ctypes.Structure):
         ("collect_balg", ctypes.c_int32),
         ("no_batch", ctypes.c_int32),
         ("no_pair", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("split_min", ctypes.c_int32),
         ("task_begin", ctypes.c_int64),
         ("task_end", ctypes.c_int64),
     ]
@@ -111,6 +111,7 @@ class Stats(ctypes.Structure):
         ("finish_p50_ms", ctypes.c_double),
         ("finish_p90_ms", ctypes.c_double),
         ("finish_max_ms", ctypes.c_double),
+        ("split_tasks", ctypes.c_uint64),
     ]
 
 

@@ -46,6 +46,15 @@
 #define GP_HASH_SLOTS_BATCH 1024
 #define GP_HASH_SLOTS_PAIR 2048
 #define GP_HASH_SLOTS_MAX 8192
+// Heavy tasks: a task whose depth-2 loop has more than split_min candidates
+// is deferred to a second launch that runs it as up to GP_SPLIT_PARTS parts
+// (part p takes candidates p, p + parts, p + 2 parts, ...), one part per
+// work fetch, so one hub edge no longer decides the kernel's length.
+#define GP_SPLIT_DEFAULT 1024     // candidates; per part the depth-1 sets are rebuilt
+#define GP_SPLIT_DEFAULT_PAIR 256 // pair-level tasks cost far more per candidate
+#define GP_SPLIT_GRAIN 64         // candidates per part, at least
+#define GP_SPLIT_PARTS 64
+#define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 
 namespace {
 
@@ -66,6 +75,10 @@ struct KArgs {
     int64_t bslot_stride;
     int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
     int32_t hash_lg;
+    int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
+    unsigned long long* heavy;     // deferred tasks: task << 24 | min(candidates, 2^24 - 1)
+    unsigned long long* heavy_cnt; // number deferred (may exceed GP_HEAVY_CAP)
+    unsigned long long* counter2;  // work counter of the parts launch
     unsigned long long* out; // [0] raw sum, [1] B_alg bytes, [2] tasks taken
     unsigned long long* clocks; // per wave: start, finish (wall clock ticks)
 };
@@ -268,7 +281,7 @@ __device__ __forceinline__ uint32_t staged_has_n(const int32_t* tab, int lg, con
     return hits;
 }
 
-template <bool FAITHFUL>
+template <bool FAITHFUL, bool PARTS>
 __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
     __shared__ int32_t s_len[GP_MAX_PREFIX];
@@ -299,6 +312,8 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     int32_t* bbase = wbase + (int64_t)P.nslots * a.slot_stride + 2 * a.tmp_stride;
     uint64_t acc = 0, balg = 0, taken = 0;
     uint64_t lacc = 0; // per-lane partial sums of the batched leaf level
+    // the depth-2 loop visits candidates dstart, dstart + dstride, ... (a part)
+    int32_t dstart = 0, dstride = 1; // stay 0 and 1 unless PARTS
 
     auto bound = [&](uint32_t mask) -> int32_t {
         int32_t b = v_cnt;
@@ -447,7 +462,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         int32_t len = s_len[q];
         if (P.rmask[d]) len = wave_lower_bound(s_ptr[q], len, bound(P.rmask[d]));
         s_ll[d] = len;
-        s_li[d] = 0;
+        s_li[d] = d == 2 ? dstart : 0;
     };
 
     // The last loop level (depth d = L-1) as one batch: every candidate v of
@@ -457,9 +472,10 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     // across the 64 lanes.  Candidates whose list is far longer than the
     // staged sets are handled by probing the sets into the list instead.
     auto leaf_level = [&](int d) {
-        const int32_t ncand = s_ll[d];
+        const int32_t cstart = d == 2 ? dstart : 0, cstep = d == 2 ? dstride : 1;
+        const int32_t ncand = s_ll[d] > cstart ? (s_ll[d] - cstart + cstep - 1) / cstep : 0;
         if (ncand <= 0) return;
-        const int32_t* cand = s_lp[d];
+        const int32_t* cand = s_lp[d] + cstart;
         const int nb = P.nblocks;
         // 1. the fixed set G of every block
         int32_t smax = -1;       // largest element of any streamed set
@@ -522,7 +538,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         for (int32_t base = 0; base < ncand; base += WAVE) {
             const int32_t ci = base + lane;
             bool valid = ci < ncand;
-            const int32_t v = valid ? cand[ci] : -1;
+            const int32_t v = valid ? cand[(int64_t)ci * cstep] : -1;
             for (int j = 0; j < d; ++j) valid = valid && v != s_emb[j];
             int32_t nvs = 0, deg = 0;
             if (valid) {
@@ -699,8 +715,9 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     // map goes through the batched leaf level on its own.
     auto pair_level = [&]() {
         const int da = L - 2, dv = L - 1;
-        const int32_t* D = s_lp[da];
-        const int32_t Dn = s_ll[da];
+        const int32_t astart = da == 2 ? dstart : 0, astep = da == 2 ? dstride : 1;
+        const int32_t* D = s_lp[da] + astart;
+        const int32_t Dn = s_ll[da] > astart ? (s_ll[da] - astart + astep - 1) / astep : 0;
         if (Dn <= 0) return;
         const int qc = P.loop_set[dv];
         const int32_t* C = s_ptr[qc];
@@ -719,7 +736,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         for (int32_t abase = 0; abase < Dn; abase += WAVE) {
             const int32_t ai = abase + lane;
             bool va = ai < Dn;
-            const int32_t av = va ? D[ai] : -1;
+            const int32_t av = va ? D[(int64_t)ai * astep] : -1;
             for (int j = 0; j < da; ++j) va = va && av != s_emb[j];
             int32_t as = 0, acut = 0;
             if (va) {
@@ -852,96 +869,126 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         }
     };
 
+    // Work loop.  First launch (PARTS = false): chunks of tasks, deferring
+    // heavy ones.  Second launch (PARTS = true): parts of the deferred tasks,
+    // one per fetch.
+    const bool may_defer = !PARTS && !FAITHFUL && !P.root_mode && a.split_min != INT_MAX;
+    const uint64_t n_items = !PARTS ? a.n_tasks : min((uint64_t)*a.heavy_cnt, (uint64_t)GP_HEAVY_CAP) * GP_SPLIT_PARTS;
+    uint64_t cur = 0, cend = 0;
     for (;;) {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
-        base = __shfl(base, 0);
-        if (base >= a.n_tasks) break;
-        const uint64_t end = min((uint64_t)base + (uint64_t)a.chunk, a.n_tasks);
-        for (uint64_t tt = base; tt < end; ++tt) {
-            const uint64_t t = a.task_base + tt;
-            ++taken;
-            int32_t v0, v1 = -1;
-            if (P.root_mode) {
-                v0 = (int32_t)t;
-            } else {
-                v0 = a.src[t];
-                v1 = a.edge[t];
-                if (v1 == v0) continue;
-                if ((P.rmask[1] & 1) && v1 >= v0) continue;
+        uint64_t t;
+        if (!PARTS) {
+            if (cur >= cend) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
+                base = __shfl(base, 0);
+                if (base >= n_items) break;
+                cur = base;
+                cend = min((uint64_t)base + (uint64_t)a.chunk, n_items);
             }
-            s_emb[0] = v0;
-            const uint32_t l0 = a.vertex[v0];
-            const int32_t* n0 = a.edge + l0;
-            const int32_t d0 = (int32_t)(a.vertex[v0 + 1] - l0);
-            for (int i = 0; i < P.nbuild[0]; ++i) build_prefix(P.build[0][i], n0, d0);
-            if (P.root_mode) {
-                if (L == 1) {
-                    acc += leaf();
-                } else {
-                    enter(1);
-                    leaf_level(1);
-                }
+            t = a.task_base + cur++;
+            ++taken;
+        } else {
+            unsigned long long w = 0;
+            if (lane == 0) w = atomicAdd(a.counter2, 1ull);
+            w = __shfl(w, 0);
+            if (w >= n_items) break;
+            const unsigned long long e = a.heavy[w / GP_SPLIT_PARTS];
+            const int32_t part = (int32_t)(w % GP_SPLIT_PARTS);
+            const int32_t parts = max(2, min(GP_SPLIT_PARTS, (int32_t)(e & 0xFFFFFF) / GP_SPLIT_GRAIN));
+            if (part >= parts) continue;
+            dstart = part;
+            dstride = parts;
+            t = e >> 24;
+        }
+        // the task: depth 0 and 1 from the edge (v0, v1), then depth 2 on
+        int32_t v0, v1 = -1;
+        if (P.root_mode) {
+            v0 = (int32_t)t;
+        } else {
+            v0 = a.src[t];
+            v1 = a.edge[t];
+            if (v1 == v0) continue;
+            if ((P.rmask[1] & 1) && v1 >= v0) continue;
+        }
+        s_emb[0] = v0;
+        const uint32_t l0 = a.vertex[v0];
+        const int32_t* n0 = a.edge + l0;
+        const int32_t d0 = (int32_t)(a.vertex[v0 + 1] - l0);
+        for (int i = 0; i < P.nbuild[0]; ++i) build_prefix(P.build[0][i], n0, d0);
+        if (P.root_mode) {
+            if (L == 1) {
+                acc += leaf();
+            } else {
+                enter(1);
+                leaf_level(1);
+            }
+            continue;
+        }
+        s_emb[1] = v1;
+        if (FAITHFUL) balg += 8;
+        const uint32_t l1 = a.vertex[v1];
+        const int32_t* n1 = a.edge + l1;
+        const int32_t d1 = (int32_t)(a.vertex[v1 + 1] - l1);
+        if (!build_level(1, n1, d1)) continue;
+        if (L == 2) {
+            acc += leaf();
+            continue;
+        }
+        enter(2);
+        if (may_defer && s_ll[2] > a.split_min) {
+            unsigned long long slot = 0;
+            if (lane == 0) slot = atomicAdd(a.heavy_cnt, 1ull);
+            slot = __shfl(slot, 0);
+            if (slot < GP_HEAVY_CAP) {
+                if (lane == 0) a.heavy[slot] = (unsigned long long)t << 24 | (unsigned long long)min(s_ll[2], 0xFFFFFF);
                 continue;
             }
-            s_emb[1] = v1;
+        }
+        if (P.pair && L == 4) {
+            pair_level();
+            continue;
+        }
+        if (P.batched && L == 3) {
+            leaf_level(2);
+            continue;
+        }
+        // depth-first walk over depths 2 .. L-1 (L-2 when batched)
+        int d = 2;
+        for (;;) {
+            const int32_t idx = s_li[d];
+            if (idx >= s_ll[d]) {
+                if (d == 2) break;
+                --d;
+                continue;
+            }
+            s_li[d] = idx + (d == 2 ? dstride : 1);
+            const int32_t v = s_lp[d][idx];
+            bool dup = false;
+            for (int j = 0; j < d; ++j) dup = dup || s_emb[j] == v;
+            if (dup) continue;
+            s_emb[d] = v;
             if (FAITHFUL) balg += 8;
-            const uint32_t l1 = a.vertex[v1];
-            const int32_t* n1 = a.edge + l1;
-            const int32_t d1 = (int32_t)(a.vertex[v1 + 1] - l1);
-            if (!build_level(1, n1, d1)) continue;
-            if (P.pair && L == 4) {
-                enter(2);
+            const uint32_t lv = a.vertex[v];
+            const int32_t* nv = a.edge + lv;
+            const int32_t dv = (int32_t)(a.vertex[v + 1] - lv);
+            if (!build_level(d, nv, dv)) continue;
+            if (P.pair && d == L - 3) {
+                enter(L - 2);
                 pair_level();
                 continue;
             }
-            if (L == 2) {
+            if (P.batched && d == L - 2) {
+                enter(L - 1);
+                leaf_level(L - 1);
+                continue;
+            }
+            if (!P.batched && d == L - 1) {
                 acc += leaf();
                 continue;
             }
-            if (P.batched && L == 3) {
-                enter(2);
-                leaf_level(2);
-                continue;
-            }
-            // depth-first walk over depths 2 .. L-1 (L-2 when batched)
-            int d = 2;
+            ++d;
             enter(d);
-            for (;;) {
-                const int32_t idx = s_li[d];
-                if (idx >= s_ll[d]) {
-                    if (d == 2) break;
-                    --d;
-                    continue;
-                }
-                s_li[d] = idx + 1;
-                const int32_t v = s_lp[d][idx];
-                bool dup = false;
-                for (int j = 0; j < d; ++j) dup = dup || s_emb[j] == v;
-                if (dup) continue;
-                s_emb[d] = v;
-                if (FAITHFUL) balg += 8;
-                const uint32_t lv = a.vertex[v];
-                const int32_t* nv = a.edge + lv;
-                const int32_t dv = (int32_t)(a.vertex[v + 1] - lv);
-                if (!build_level(d, nv, dv)) continue;
-                if (P.pair && d == L - 3) {
-                    enter(L - 2);
-                    pair_level();
-                    continue;
-                }
-                if (P.batched && d == L - 2) {
-                    enter(L - 1);
-                    leaf_level(L - 1);
-                    continue;
-                }
-                if (!P.batched && d == L - 1) {
-                    acc += leaf();
-                    continue;
-                }
-                ++d;
-                enter(d);
-            }
         }
     }
     const uint64_t lsum = wave_sum_u64(lacc);
@@ -984,7 +1031,8 @@ struct CallCtx {
     unsigned long long* d_clocks = nullptr;
     size_t clocks_cap = 0; // waves
     KPlan* d_plan = nullptr;
-    unsigned long long* d_cnt = nullptr; // [0] task counter, [1..3] outputs
+    unsigned long long* d_cnt = nullptr; // [0] task counter, [1..3] outputs, [4] part counter, [5] deferred
+    unsigned long long* d_heavy = nullptr; // GP_HEAVY_CAP deferred tasks
     int32_t* d_scratch = nullptr;
     size_t scratch_bytes = 0;
     hipStream_t stream = nullptr;
@@ -1018,6 +1066,7 @@ void destroy_ctx(CallCtx* c) {
     if (c->d_plan) (void)hipFree(c->d_plan);
     if (c->d_clocks) (void)hipFree(c->d_clocks);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
+    if (c->d_heavy) (void)hipFree(c->d_heavy);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
@@ -1038,7 +1087,8 @@ int acquire_ctx(gp_graph* g, size_t scratch_bytes, CallCtx** out) {
         c = new CallCtx();
         hipError_t e = hipSuccess;
         if (e == hipSuccess) e = hipMalloc((void**)&c->d_plan, sizeof(KPlan));
-        if (e == hipSuccess) e = hipMalloc((void**)&c->d_cnt, 4 * sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipMalloc((void**)&c->d_cnt, 8 * sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipMalloc((void**)&c->d_heavy, GP_HEAVY_CAP * sizeof(unsigned long long));
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreate(&c->e0);
         if (e == hipSuccess) e = hipEventCreate(&c->e1);
@@ -1123,7 +1173,8 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
         if (c->d_clocks) (void)hipFree(c->d_clocks);
         c->d_clocks = nullptr;
         c->clocks_cap = 0;
-        hipError_t ce = hipMalloc((void**)&c->d_clocks, sizeof(unsigned long long) * 2 * (size_t)waves);
+        // start and finish stamps per wave, for each of the two launches
+        hipError_t ce = hipMalloc((void**)&c->d_clocks, sizeof(unsigned long long) * 4 * (size_t)waves);
         if (ce != hipSuccess) {
             destroy_ctx(c);
             return gp_fail(GP_ENOMEM, std::string("clock buffer: ") + hipGetErrorString(ce));
@@ -1149,26 +1200,40 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     a.hash_lg = hash_lg;
     a.out = c->d_cnt + 1;
     a.clocks = c->d_clocks;
-    unsigned long long host_out[4] = {0, 0, 0, 0};
-    std::vector<unsigned long long> clocks(2 * (size_t)waves, 0);
+    // heavy-task splitting: only where a depth-2 loop exists
+    const int split_req = tuning ? tuning->split_min : 0;
+    const bool split = !faithful && !kp.root_mode && kp.leaf >= 3 && split_req >= 0;
+    a.split_min = split ? (split_req > 0 ? split_req : kp.pair ? GP_SPLIT_DEFAULT_PAIR : GP_SPLIT_DEFAULT) : INT_MAX;
+    a.heavy = c->d_heavy;
+    a.heavy_cnt = c->d_cnt + 5;
+    a.counter2 = c->d_cnt + 4;
+    unsigned long long host_out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<unsigned long long> clocks(4 * (size_t)waves, 0);
     hipError_t e = hipMemcpyAsync(c->d_plan, &kp, sizeof(KPlan), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt, 4);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt, 8);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
     if (e == hipSuccess && n_tasks > 0) {
         if (faithful)
-            hipLaunchKernelGGL(gp_count_kernel<true>, dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
+            hipLaunchKernelGGL((gp_count_kernel<true, false>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
         else
-            hipLaunchKernelGGL(gp_count_kernel<false>, dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
+            hipLaunchKernelGGL((gp_count_kernel<false, false>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
         e = hipGetLastError();
+        if (e == hipSuccess && split) {
+            // the deferred tasks, in parts; the kernel reads how many there are
+            KArgs a2 = a;
+            a2.clocks = c->d_clocks + 2 * waves;
+            hipLaunchKernelGGL((gp_count_kernel<false, true>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a2);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && n_tasks > 0)
-        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * clocks.size(), hipMemcpyDeviceToHost,
-                           c->stream);
+        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * (split ? 4 : 2) * (size_t)waves,
+                           hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     float ms = 0;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
@@ -1189,6 +1254,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
                                            (kp.root_mode ? 8ull * n_tasks : 0)
                                      : 0;
         stats->tasks = host_out[3];
+        stats->split_tasks = split ? std::min<uint64_t>(host_out[5], GP_HEAVY_CAP) : 0;
         stats->waves = (int32_t)waves;
         stats->kernel_variant = faithful ? 1 : 0;
         stats->finish_p50_ms = stats->finish_p90_ms = stats->finish_max_ms = 0;
@@ -1200,7 +1266,11 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
             unsigned long long t0 = ~0ull;
             for (int64_t w = 0; w < waves; ++w) t0 = std::min(t0, clocks[2 * w]);
             std::vector<double> fin((size_t)waves);
-            for (int64_t w = 0; w < waves; ++w) fin[w] = (double)(clocks[2 * w + 1] - t0) / rate_khz;
+            for (int64_t w = 0; w < waves; ++w) {
+                unsigned long long f = clocks[2 * w + 1];
+                if (split) f = std::max(f, clocks[2 * waves + 2 * w + 1]);
+                fin[w] = (double)(f - t0) / rate_khz;
+            }
             std::sort(fin.begin(), fin.end());
             stats->finish_p50_ms = fin[fin.size() / 2];
             stats->finish_p90_ms = fin[fin.size() * 9 / 10];

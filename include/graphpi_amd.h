@@ -95,7 +95,10 @@ typedef struct gp_tuning {
     int32_t collect_balg;   /* 1 = run the reference-faithful loop nest and count B_alg */
     int32_t no_batch;       /* 1 = evaluate the last loop level one candidate at a time */
     int32_t no_pair;        /* 1 = never evaluate the last two loop levels together */
-    int32_t reserved;
+    int32_t split_min;      /* a task whose depth-2 loop has more candidates than this is
+                               deferred and run as up to 64 parts in a second launch;
+                               0 = default (1024; 256 when the last two loop levels run
+                               as a pair), < 0 = never split */
     /* Count only tasks [task_begin, task_end) (task_end <= 0: to the end).
      * Tasks are directed CSR edge slots, or root vertices when the pattern's
      * leaf is at depth 1 (gp_count_tasks gives the total).  Raw sums of
@@ -114,6 +117,7 @@ typedef struct gp_stats {
     double finish_p50_ms;   /* time after kernel start by which half the waves had finished */
     double finish_p90_ms;   /* ... 90% of the waves */
     double finish_max_ms;   /* ... every wave (load-balance tail) */
+    uint64_t split_tasks;   /* tasks deferred and run in parts */
 } gp_stats;
 
 typedef struct gp_graph gp_graph; /* device-resident CSR (opaque) */

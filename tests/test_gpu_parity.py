@@ -180,6 +180,38 @@ def test_task_ranges_add_up(rmat13_path):
         dev.count(s, bad)
 
 
+def test_split_heavy_tasks_same_counts(rmat13_path):
+    # split_min = 1 defers almost every task to the parts launch (strided
+    # depth-2 candidates); -1 never splits; 0 is the default threshold
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    tuns = {}
+    for v in (1, 40, -1):
+        tuns[v] = gp.Tuning()
+        tuns[v].split_min = v
+    no_batch = gp.Tuning()
+    no_batch.no_batch = 1
+    no_batch.split_min = 1
+    cases = [(name, (1, 1, 1)) for name in ["triangle", "rectangle", "house", "4clique", "5clique", "pentagon",
+                                             "qg3", "cycle_6_tri", "5path"]]
+    cases += [(name, (0, 1, 0)) for name in ["house", "pentagon", "4clique"]]
+    for name, (pm, rt, iep) in cases:
+        s = _sched(host, gp.PATTERNS[name], pm, rt, iep, tri)
+        base = dev.count(s, tuns[-1])
+        assert base.stats.split_tasks == 0
+        for v in (1, 40):
+            r = dev.count(s, tuns[v])
+            print(f"{name} ({pm},{rt},{iep}) split_min={v}: {r.stats.split_tasks} split tasks, "
+                  f"{r.stats.kernel_ms:.2f} ms vs {base.stats.kernel_ms:.2f} ms", flush=True)
+            assert r.raw == base.raw, (name, pm, rt, iep, v)
+        assert dev.count(s).raw == base.raw
+        assert dev.count(s, no_batch).raw == base.raw, (name, "no_batch")
+    # with task ranges: the deferred tasks of each range stay in that range
+    s = _sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)
+    assert dev.count_sliced(s, 9, tuns[1]).raw == dev.count(s, tuns[-1]).raw
+
+
 def test_no_truncation_same_counts(rmat13_path):
     host = gp.HostGraph.load_text(rmat13_path)
     dev = gp.DeviceGraph(host)
