@@ -139,6 +139,9 @@ _SIGS = {
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(Stats)]),
     "gp_count_tasks": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(ctypes.c_uint64)]),
+    "gp_count_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
+                                      ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]),
     "gp_tc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]),
     "gp_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
     "gp_last_error": (ctypes.c_char_p, []),
@@ -378,6 +381,18 @@ class DeviceGraph:
         _check(lib().gp_count(self._h, ctypes.byref(sched), ctypes.byref(tuning) if tuning else None,
                               ctypes.byref(raw), ctypes.byref(cnt), ctypes.byref(st)))
         return CountResult(cnt.value, raw.value, st)
+
+    def count_batch(self, scheds, tuning=None):
+        """gp_count_batch: several schedules in one persistent launch.
+        Returns ([CountResult per schedule], Stats of the whole batch)."""
+        n = len(scheds)
+        arr = (SchedFlat * n)(*scheds)
+        raws = (ctypes.c_uint64 * n)()
+        cnts = (ctypes.c_int64 * n)()
+        st = Stats()
+        _check(lib().gp_count_batch(self._h, arr, n, ctypes.byref(tuning) if tuning else None, raws, cnts,
+                                    ctypes.byref(st)))
+        return [CountResult(cnts[i], raws[i], None) for i in range(n)], st
 
     def count_tasks(self, sched):
         t = ctypes.c_uint64()

@@ -55,18 +55,22 @@
 #define GP_SPLIT_GRAIN 64         // candidates per part, at least
 #define GP_SPLIT_PARTS 64
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
+#define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 
 namespace {
 
 struct KArgs {
-    const KPlan* plan;
+    const KPlan* plans;       // one per schedule counted by this launch
+    int32_t nplans;
+    const uint64_t* task_off;   // plan p owns work items [task_off[p], task_off[p+1]), a multiple of chunk
+    const uint64_t* task_first; // first task (edge slot or root) of plan p
+    const uint64_t* task_cnt;   // tasks of plan p (its items past that are padding)
     const uint32_t* vertex;
     const int32_t* edge;
     const int32_t* src;
     int32_t v_cnt;
     int32_t chunk;
-    uint64_t n_tasks;
-    uint64_t task_base;
+    uint64_t n_tasks;         // task_off[nplans]
     unsigned long long* counter;
     int32_t* scratch;
     int64_t slot_stride;
@@ -76,10 +80,10 @@ struct KArgs {
     int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
     int32_t hash_lg;
     int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
-    unsigned long long* heavy;     // deferred tasks: task << 24 | min(candidates, 2^24 - 1)
+    unsigned long long* heavy;     // deferred: plan << 56 | task << 24 | min(candidates, 2^24 - 1)
     unsigned long long* heavy_cnt; // number deferred (may exceed GP_HEAVY_CAP)
     unsigned long long* counter2;  // work counter of the parts launch
-    unsigned long long* out; // [0] raw sum, [1] B_alg bytes, [2] tasks taken
+    unsigned long long* out; // per plan p: [3p] raw sum, [3p+1] B_alg bytes, [3p+2] tasks taken
     unsigned long long* clocks; // per wave: start, finish (wall clock ticks)
 };
 
@@ -95,6 +99,14 @@ typedef int32_t __attribute__((address_space(1)))* gptr;
 #define GLW(p) ((gptr)(p))
 
 __device__ __forceinline__ int popc64(uint64_t m) { return __popcll(m); }
+
+// lane 0's value as a wave-uniform (scalar) value: lets the compiler keep
+// what is derived from it, such as the current plan, in scalar registers
+__device__ __forceinline__ uint64_t first_lane_u64(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return (uint64_t)hi << 32 | lo;
+}
 
 // sorted-list membership, one lane one probe sequence
 __device__ __forceinline__ bool contains(const int32_t* __restrict__ ap, int32_t n, int32_t x) {
@@ -282,7 +294,7 @@ __device__ __forceinline__ uint32_t staged_has_n(const int32_t* tab, int lg, con
 }
 
 template <bool FAITHFUL, bool PARTS>
-__global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
+__global__ void __launch_bounds__(WAVE, 5) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
     __shared__ int32_t s_len[GP_MAX_PREFIX];
     __shared__ int32_t s_emb[GP_MAX_SIZE];
@@ -303,13 +315,23 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     __shared__ int32_t s_off[WAVE + 1];
     __shared__ int32_t s_off2[WAVE + 1];
 
-    const KPlan& P = *a.plan;
+    // the plan of the current work item; switching plans flushes the sums
+    int cur_plan = 0;
+    const KPlan* pl = a.plans;
     const int lane = lane_id();
     if (lane == 0) a.clocks[2 * blockIdx.x] = wall_clock64();
-    const int n = P.n, L = P.leaf, K = P.k;
+    int n = pl->n, L = pl->leaf, K = pl->k;
     const int32_t v_cnt = a.v_cnt;
     int32_t* wbase = a.scratch + (int64_t)blockIdx.x * a.wave_stride;
-    int32_t* bbase = wbase + (int64_t)P.nslots * a.slot_stride + 2 * a.tmp_stride;
+    int32_t* bbase = wbase + (int64_t)pl->nslots * a.slot_stride + 2 * a.tmp_stride;
+    auto set_plan = [&](int p) {
+        cur_plan = p;
+        pl = a.plans + p;
+        n = pl->n;
+        L = pl->leaf;
+        K = pl->k;
+        bbase = wbase + (int64_t)pl->nslots * a.slot_stride + 2 * a.tmp_stride;
+    };
     uint64_t acc = 0, balg = 0, taken = 0;
     uint64_t lacc = 0; // per-lane partial sums of the batched leaf level
     // the depth-2 loop visits candidates dstart, dstart + dstride, ... (a part)
@@ -322,13 +344,13 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         return b;
     };
     auto ub_of = [&](int p) -> int32_t {
-        if (FAITHFUL || P.ub_n[p] == 0) return INT_MAX;
+        if (FAITHFUL || pl->ub_n[p] == 0) return INT_MAX;
         int32_t ub = 0;
-        for (int i = 0; i < P.ub_n[p]; ++i) ub = max(ub, bound(P.ub_mask[p][i]));
+        for (int i = 0; i < pl->ub_n[p]; ++i) ub = max(ub, bound(pl->ub_mask[p][i]));
         return ub;
     };
     auto build_prefix = [&](int p, const int32_t* nv, int32_t deg) -> int32_t {
-        const int f = P.father[p];
+        const int f = pl->father[p];
         const int32_t ub = ub_of(p);
         const int32_t* ptr;
         int32_t len;
@@ -336,7 +358,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             ptr = nv;
             len = ub == INT_MAX ? deg : wave_lower_bound(nv, deg, ub);
         } else {
-            int32_t* out = wbase + (int64_t)P.slot[p] * a.slot_stride;
+            int32_t* out = wbase + (int64_t)pl->slot[p] * a.slot_stride;
             const int32_t* fp = s_ptr[f];
             const int32_t fl = s_len[f];
             if (FAITHFUL) balg += 4ull * (uint64_t)((int64_t)fl + deg);
@@ -350,8 +372,8 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     };
     // build the prefixes whose last vertex is depth d; false when one is empty
     auto build_level = [&](int d, const int32_t* nv, int32_t deg) -> bool {
-        for (int i = 0; i < P.nbuild[d]; ++i)
-            if (build_prefix(P.build[d][i], nv, deg) == 0) return false;
+        for (int i = 0; i < pl->nbuild[d]; ++i)
+            if (build_prefix(pl->build[d][i], nv, deg) == 0) return false;
         return true;
     };
     // count of the common elements of the block's sets, minus embedded ones
@@ -360,7 +382,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         int32_t best_len = INT_MAX;
         for (int i = 0; i < K; ++i)
             if (bm >> i & 1) {
-                int q = P.loop_set[L + i];
+                int q = pl->loop_set[L + i];
                 if (s_len[q] < best_len) {
                     best_len = s_len[q];
                     best = q;
@@ -374,7 +396,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             int32_t x = hit ? S[i] : 0;
             for (int t = 0; t < K; ++t) {
                 if (!(bm >> t & 1)) continue;
-                int q = P.loop_set[L + t];
+                int q = pl->loop_set[L + t];
                 if (q == best) continue;
                 hit = hit && contains(s_ptr[q], s_len[q], x);
             }
@@ -386,20 +408,20 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
     // count at depth L with emb[0:L) placed and every set built (one candidate)
     auto leaf = [&]() -> uint64_t {
         if (K == 0) {
-            const int q = P.loop_set[L];
-            const int32_t b = bound(P.rmask[L]);
+            const int q = pl->loop_set[L];
+            const int32_t b = bound(pl->rmask[L]);
             const int32_t* S = s_ptr[q];
             int32_t c = s_len[q];
-            if (P.rmask[L]) c = wave_lower_bound(S, c, b);
+            if (pl->rmask[L]) c = wave_lower_bound(S, c, b);
             if (c > 0) c -= count_emb_in(S, c, s_emb, L);
             return (uint64_t)(int64_t)c;
         }
         if (!FAITHFUL) {
-            for (int bi = 0; bi < P.nblocks; ++bi) {
-                const uint32_t bm = P.block_mask[bi];
+            for (int bi = 0; bi < pl->nblocks; ++bi) {
+                const uint32_t bm = pl->block_mask[bi];
                 int32_t c;
                 if (__popc(bm) == 1) {
-                    const int q = P.loop_set[L + __ffs(bm) - 1];
+                    const int q = pl->loop_set[L + __ffs(bm) - 1];
                     c = s_len[q] - count_emb_in(s_ptr[q], s_len[q], s_emb, L);
                 } else {
                     c = block_count(bm);
@@ -407,25 +429,25 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 s_blk[bi] = c;
             }
             uint64_t sum = 0;
-            for (int g = 0; g < P.ngroups; ++g) {
-                uint64_t val = (uint64_t)P.group_val[g];
-                for (int j = 0; j < P.group_nb[g]; ++j) val *= (uint64_t)(int64_t)s_blk[P.group_blk[g][j]];
+            for (int g = 0; g < pl->ngroups; ++g) {
+                uint64_t val = (uint64_t)pl->group_val[g];
+                for (int j = 0; j < pl->group_nb[g]; ++j) val *= (uint64_t)(int64_t)s_blk[pl->group_blk[g][j]];
                 sum += val;
             }
             return sum;
         }
         // reference order: group by group, block by block, stop at zero
         uint64_t sum = 0;
-        int32_t* tmp0 = wbase + (int64_t)P.nslots * a.slot_stride;
+        int32_t* tmp0 = wbase + (int64_t)pl->nslots * a.slot_stride;
         int32_t* tmp1 = tmp0 + a.tmp_stride;
-        if (s_len[P.loop_set[L]] <= 0) return 0; // src/graph.cpp:250-252
-        for (int g = 0; g < P.ngroups; ++g) {
-            uint64_t val = (uint64_t)P.group_val[g];
-            for (int j = 0; j < P.group_nb[g]; ++j) {
-                const uint32_t bm = P.block_mask[P.group_blk[g][j]];
+        if (s_len[pl->loop_set[L]] <= 0) return 0; // src/graph.cpp:250-252
+        for (int g = 0; g < pl->ngroups; ++g) {
+            uint64_t val = (uint64_t)pl->group_val[g];
+            for (int j = 0; j < pl->group_nb[g]; ++j) {
+                const uint32_t bm = pl->block_mask[pl->group_blk[g][j]];
                 int32_t c;
                 if (__popc(bm) == 1) {
-                    const int q = P.loop_set[L + __ffs(bm) - 1];
+                    const int q = pl->loop_set[L + __ffs(bm) - 1];
                     c = s_len[q] - count_emb_in(s_ptr[q], s_len[q], s_emb, L);
                 } else {
                     int first = -1;
@@ -434,7 +456,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                     int32_t* dst = tmp0;
                     for (int t = 0; t < K; ++t) {
                         if (!(bm >> t & 1)) continue;
-                        const int q = P.loop_set[L + t];
+                        const int q = pl->loop_set[L + t];
                         if (first < 0) {
                             first = q;
                             continue;
@@ -457,10 +479,10 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         return sum;
     };
     auto enter = [&](int d) {
-        const int q = P.loop_set[d];
+        const int q = pl->loop_set[d];
         s_lp[d] = s_ptr[q];
         int32_t len = s_len[q];
-        if (P.rmask[d]) len = wave_lower_bound(s_ptr[q], len, bound(P.rmask[d]));
+        if (pl->rmask[d]) len = wave_lower_bound(s_ptr[q], len, bound(pl->rmask[d]));
         s_ll[d] = len;
         s_li[d] = d == 2 ? dstart : 0;
     };
@@ -476,7 +498,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         const int32_t ncand = s_ll[d] > cstart ? (s_ll[d] - cstart + cstep - 1) / cstep : 0;
         if (ncand <= 0) return;
         const int32_t* cand = s_lp[d] + cstart;
-        const int nb = P.nblocks;
+        const int nb = pl->nblocks;
         // 1. the fixed set G of every block
         int32_t smax = -1;       // largest element of any streamed set
         int nstream = 0;
@@ -487,19 +509,19 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         int32_t stream_total = 0;
         int hoff = 0;
         for (int b = 0; b < nb; ++b) {
-            const int kind = P.bl_kind[b];
+            const int kind = pl->bl_kind[b];
             const int32_t* g = nullptr;
             int32_t gn = 0;
             if (kind != 2) {
-                const int ns = P.bl_nsrc[b];
-                const int p0 = P.bl_src[b][0];
+                const int ns = pl->bl_nsrc[b];
+                const int p0 = pl->bl_src[b][0];
                 g = s_ptr[p0];
                 gn = s_len[p0];
                 if (ns >= 2) {
-                    int32_t* out = bbase + (int64_t)P.bl_slot[b] * a.bslot_stride;
-                    int32_t* spare = bbase + (int64_t)P.nbslots * a.bslot_stride;
+                    int32_t* out = bbase + (int64_t)pl->bl_slot[b] * a.bslot_stride;
+                    int32_t* spare = bbase + (int64_t)pl->nbslots * a.bslot_stride;
                     for (int i = 1; i < ns; ++i) {
-                        const int pi = P.bl_src[b][i];
+                        const int pi = pl->bl_src[b][i];
                         // alternate so the last intersection lands in 'out'
                         int32_t* dst = ((ns - 1 - i) & 1) ? spare : out;
                         gn = intersect<false>(g, gn, s_ptr[pi], s_len[pi], INT_MAX, dst, nullptr, 0);
@@ -533,7 +555,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             }
         }
         const bool plain = K == 0;
-        const uint32_t leaf_mask = plain ? P.rmask[L] : 0;
+        const uint32_t leaf_mask = plain ? pl->rmask[L] : 0;
         // 2. candidates in batches of 64
         for (int32_t base = 0; base < ncand; base += WAVE) {
             const int32_t ci = base + lane;
@@ -571,7 +593,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             const int32_t lwork = work;
             if (longc) work = 0;
             for (int b2 = 0; b2 < nb; ++b2)
-                if (P.bl_kind[b2] == 1) s_bc[b2][lane] = 0;
+                if (pl->bl_kind[b2] == 1) s_bc[b2][lane] = 0;
             int32_t total = 0;
             const int32_t off = wave_excl_scan(work, &total);
             s_off[lane] = off;
@@ -600,7 +622,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 const int next = above ? __ffsll((long long)above) - 1 : WAVE;
                 const uint64_t seg = (next == WAVE ? ~0ull : ((1ull << next) - 1)) & (~0ull << lane);
                 for (int b2 = 0; b2 < nb; ++b2) {
-                    if (P.bl_kind[b2] != 1) continue;
+                    if (pl->bl_kind[b2] != 1) continue;
                     bool hit = false;
                     if (keep) {
                         const int h = s_bh[b2];
@@ -625,7 +647,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 const gcptr pv = GL(a.edge) + __shfl(nvs, j);
                 const int32_t pn = __shfl(lwork, j);
                 for (int b2 = 0; b2 < nb; ++b2) {
-                    if (P.bl_kind[b2] != 1) continue;
+                    if (pl->bl_kind[b2] != 1) continue;
                     const int h = s_bh[b2];
                     const int lg = s_blg[b2];
                     const int32_t* g = s_bg[b2];
@@ -664,7 +686,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                 const int32_t pd = __shfl(deg, j);
                 const int32_t pc = __shfl(scut, j);
                 for (int b2 = 0; b2 < nb; ++b2) {
-                    if (P.bl_kind[b2] != 1) continue;
+                    if (pl->bl_kind[b2] != 1) continue;
                     const int32_t c = intersect<true>(s_bg[b2], s_bn[b2], pv, pd, pc, nullptr, s_emb, d);
                     if (lane == j) s_bc[b2][j] = c;
                 }
@@ -674,7 +696,7 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             if (valid) {
                 // kind-1 counts are already in s_bc; fill in the others
                 for (int b2 = 0; b2 < nb; ++b2) {
-                    const int kind = P.bl_kind[b2];
+                    const int kind = pl->bl_kind[b2];
                     if (kind == 2) {
                         // |N(v) ∩ [0, b) \ emb|
                         const int32_t cut = b == INT_MAX ? deg : lower_bound_lane(nv, deg, b);
@@ -695,9 +717,9 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
                     }
                 }
                 uint64_t sum = 0;
-                for (int g = 0; g < P.ngroups; ++g) {
-                    uint64_t val = (uint64_t)P.group_val[g];
-                    for (int j = 0; j < P.group_nb[g]; ++j) val *= (uint64_t)(int64_t)s_bc[P.group_blk[g][j]][lane];
+                for (int g = 0; g < pl->ngroups; ++g) {
+                    uint64_t val = (uint64_t)pl->group_val[g];
+                    for (int j = 0; j < pl->group_nb[g]; ++j) val *= (uint64_t)(int64_t)s_bc[pl->group_blk[g][j]][lane];
                     sum += val;
                 }
                 lacc += sum;
@@ -719,13 +741,13 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         const int32_t* D = s_lp[da] + astart;
         const int32_t Dn = s_ll[da] > astart ? (s_ll[da] - astart + astep - 1) / astep : 0;
         if (Dn <= 0) return;
-        const int qc = P.loop_set[dv];
+        const int qc = pl->loop_set[dv];
         const int32_t* C = s_ptr[qc];
         int32_t Cn = s_len[qc];
-        if (P.rmask[dv]) Cn = wave_lower_bound(C, Cn, bound(P.rmask[dv]));
+        if (pl->rmask[dv]) Cn = wave_lower_bound(C, Cn, bound(pl->rmask[dv]));
         if (Cn <= 0) return;
-        const int32_t b = bound(P.rmask[L]);
-        const int gpf = P.pair_g;
+        const int32_t b = bound(pl->rmask[L]);
+        const int gpf = pl->pair_g;
         const int32_t* gptr = gpf >= 0 ? s_ptr[gpf] : nullptr;
         const int32_t glen = gpf >= 0 ? s_len[gpf] : 0;
         const int32_t pair_slots = a.hash_slots >> 1;
@@ -869,10 +891,21 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
         }
     };
 
+    // add this wave's sums for the current plan to its outputs and reset them
+    auto flush = [&]() {
+        const uint64_t lsum = wave_sum_u64(lacc);
+        if (lane == 0) {
+            unsigned long long* o = a.out + 3 * cur_plan;
+            if (acc + lsum) atomicAdd(&o[0], (unsigned long long)(acc + lsum));
+            if (FAITHFUL && balg) atomicAdd(&o[1], (unsigned long long)balg);
+            if (taken) atomicAdd(&o[2], (unsigned long long)taken);
+        }
+        acc = lacc = balg = taken = 0;
+    };
+
     // Work loop.  First launch (PARTS = false): chunks of tasks, deferring
     // heavy ones.  Second launch (PARTS = true): parts of the deferred tasks,
     // one per fetch.
-    const bool may_defer = !PARTS && !FAITHFUL && !P.root_mode && a.split_min != INT_MAX;
     const uint64_t n_items = !PARTS ? a.n_tasks : min((uint64_t)*a.heavy_cnt, (uint64_t)GP_HEAVY_CAP) * GP_SPLIT_PARTS;
     uint64_t cur = 0, cend = 0;
     for (;;) {
@@ -881,42 +914,57 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             if (cur >= cend) {
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
-                base = __shfl(base, 0);
+                base = first_lane_u64(base);
                 if (base >= n_items) break;
+                // the plan owning this chunk; a chunk never spans two plans
+                int p = 0;
+                while (p + 1 < a.nplans && a.task_off[p + 1] <= base) ++p;
+                if (p != cur_plan) {
+                    flush();
+                    set_plan(p);
+                }
                 cur = base;
-                cend = min((uint64_t)base + (uint64_t)a.chunk, n_items);
+                cend = min((uint64_t)base + (uint64_t)a.chunk, (uint64_t)(a.task_off[p] + a.task_cnt[p]));
+                if (cur >= cend) continue; // padding
+
             }
-            t = a.task_base + cur++;
+            t = a.task_first[cur_plan] + (cur - a.task_off[cur_plan]);
+            ++cur;
             ++taken;
         } else {
             unsigned long long w = 0;
             if (lane == 0) w = atomicAdd(a.counter2, 1ull);
-            w = __shfl(w, 0);
+            w = first_lane_u64(w);
             if (w >= n_items) break;
             const unsigned long long e = a.heavy[w / GP_SPLIT_PARTS];
             const int32_t part = (int32_t)(w % GP_SPLIT_PARTS);
             const int32_t parts = max(2, min(GP_SPLIT_PARTS, (int32_t)(e & 0xFFFFFF) / GP_SPLIT_GRAIN));
             if (part >= parts) continue;
+            const int p = (int)(e >> 56);
+            if (p != cur_plan) {
+                flush();
+                set_plan(p);
+            }
             dstart = part;
             dstride = parts;
-            t = e >> 24;
+            t = (e >> 24) & 0xFFFFFFFFull;
         }
         // the task: depth 0 and 1 from the edge (v0, v1), then depth 2 on
         int32_t v0, v1 = -1;
-        if (P.root_mode) {
+        if (pl->root_mode) {
             v0 = (int32_t)t;
         } else {
             v0 = a.src[t];
             v1 = a.edge[t];
             if (v1 == v0) continue;
-            if ((P.rmask[1] & 1) && v1 >= v0) continue;
+            if ((pl->rmask[1] & 1) && v1 >= v0) continue;
         }
         s_emb[0] = v0;
         const uint32_t l0 = a.vertex[v0];
         const int32_t* n0 = a.edge + l0;
         const int32_t d0 = (int32_t)(a.vertex[v0 + 1] - l0);
-        for (int i = 0; i < P.nbuild[0]; ++i) build_prefix(P.build[0][i], n0, d0);
-        if (P.root_mode) {
+        for (int i = 0; i < pl->nbuild[0]; ++i) build_prefix(pl->build[0][i], n0, d0);
+        if (pl->root_mode) {
             if (L == 1) {
                 acc += leaf();
             } else {
@@ -936,20 +984,22 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             continue;
         }
         enter(2);
-        if (may_defer && s_ll[2] > a.split_min) {
+        if (!PARTS && !FAITHFUL && s_ll[2] > pl->split_min) {
             unsigned long long slot = 0;
             if (lane == 0) slot = atomicAdd(a.heavy_cnt, 1ull);
             slot = __shfl(slot, 0);
             if (slot < GP_HEAVY_CAP) {
-                if (lane == 0) a.heavy[slot] = (unsigned long long)t << 24 | (unsigned long long)min(s_ll[2], 0xFFFFFF);
+                if (lane == 0)
+                    a.heavy[slot] = (unsigned long long)cur_plan << 56 | (unsigned long long)t << 24 |
+                                    (unsigned long long)min(s_ll[2], 0xFFFFFF);
                 continue;
             }
         }
-        if (P.pair && L == 4) {
+        if (pl->pair && L == 4) {
             pair_level();
             continue;
         }
-        if (P.batched && L == 3) {
+        if (pl->batched && L == 3) {
             leaf_level(2);
             continue;
         }
@@ -973,17 +1023,17 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             const int32_t* nv = a.edge + lv;
             const int32_t dv = (int32_t)(a.vertex[v + 1] - lv);
             if (!build_level(d, nv, dv)) continue;
-            if (P.pair && d == L - 3) {
+            if (pl->pair && d == L - 3) {
                 enter(L - 2);
                 pair_level();
                 continue;
             }
-            if (P.batched && d == L - 2) {
+            if (pl->batched && d == L - 2) {
                 enter(L - 1);
                 leaf_level(L - 1);
                 continue;
             }
-            if (!P.batched && d == L - 1) {
+            if (!pl->batched && d == L - 1) {
                 acc += leaf();
                 continue;
             }
@@ -991,13 +1041,8 @@ __global__ void __launch_bounds__(WAVE) gp_count_kernel(KArgs a) {
             enter(d);
         }
     }
-    const uint64_t lsum = wave_sum_u64(lacc);
+    flush();
     if (lane == 0) a.clocks[2 * blockIdx.x + 1] = wall_clock64();
-    if (lane == 0) {
-        atomicAdd(&a.out[0], (unsigned long long)(acc + lsum));
-        if (FAITHFUL) atomicAdd(&a.out[1], (unsigned long long)balg);
-        atomicAdd(&a.out[2], (unsigned long long)taken);
-    }
 }
 
 // src[e] = the vertex whose list holds edge slot e
@@ -1017,8 +1062,7 @@ __global__ void build_src_kernel(const uint32_t* __restrict__ vertex, int32_t v_
 }
 
 __global__ void fill_u64_kernel(unsigned long long* p, int n) {
-    int i = (int)threadIdx.x;
-    if (i < n) p[i] = 0;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0;
 }
 
 } // namespace
@@ -1030,8 +1074,12 @@ __global__ void fill_u64_kernel(unsigned long long* p, int n) {
 struct CallCtx {
     unsigned long long* d_clocks = nullptr;
     size_t clocks_cap = 0; // waves
-    KPlan* d_plan = nullptr;
-    unsigned long long* d_cnt = nullptr; // [0] task counter, [1..3] outputs, [4] part counter, [5] deferred
+    // plans, then task_off[nplans + 1] and task_first[nplans] (uint64)
+    char* d_meta = nullptr;
+    size_t meta_cap = 0;
+    // [0] task counter, [1] part counter, [2] deferred tasks, then 3 outputs per plan
+    unsigned long long* d_cnt = nullptr;
+    size_t cnt_cap = 0;
     unsigned long long* d_heavy = nullptr; // GP_HEAVY_CAP deferred tasks
     int32_t* d_scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -1063,7 +1111,7 @@ namespace {
 
 void destroy_ctx(CallCtx* c) {
     if (!c) return;
-    if (c->d_plan) (void)hipFree(c->d_plan);
+    if (c->d_meta) (void)hipFree(c->d_meta);
     if (c->d_clocks) (void)hipFree(c->d_clocks);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_heavy) (void)hipFree(c->d_heavy);
@@ -1086,8 +1134,6 @@ int acquire_ctx(gp_graph* g, size_t scratch_bytes, CallCtx** out) {
     if (!c) {
         c = new CallCtx();
         hipError_t e = hipSuccess;
-        if (e == hipSuccess) e = hipMalloc((void**)&c->d_plan, sizeof(KPlan));
-        if (e == hipSuccess) e = hipMalloc((void**)&c->d_cnt, 8 * sizeof(unsigned long long));
         if (e == hipSuccess) e = hipMalloc((void**)&c->d_heavy, GP_HEAVY_CAP * sizeof(unsigned long long));
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreate(&c->e0);
@@ -1118,79 +1164,140 @@ void release_ctx(gp_graph* g, CallCtx* c) {
     g->idle.push_back(c);
 }
 
-int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw,
+// grow a device buffer to at least `bytes`
+hipError_t ensure(void** p, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) *cap = bytes;
+    return e;
+}
+
+// Count nplans schedules over one graph in one persistent launch (plus the
+// launch of deferred parts).  Results are per schedule; a task range in the
+// tuning is only accepted for a single schedule.
+int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, uint64_t* raw,
               int64_t* count, gp_stats* stats) {
+    if (nplans < 1 || nplans > GP_MAX_BATCH) return gp_fail(GP_ERANGE, "schedule count out of range");
     const bool faithful = tuning && tuning->collect_balg;
-    KPlan kp;
-    int rc = gp_make_kplan(sched, faithful, !(tuning && tuning->no_batch), !(tuning && tuning->no_pair), &kp);
-    if (rc != GP_OK) return rc;
-    if (tuning && tuning->no_truncate) {
-        std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
+    std::vector<KPlan> kps((size_t)nplans);
+    const int split_req = tuning ? tuning->split_min : 0;
+    bool any_split = false, any_batched = false, any_k = false;
+    int hash_slots = 0, bc_rows = 0, nbslots = 0;
+    int max_slots = 0;
+    for (int p = 0; p < nplans; ++p) {
+        KPlan& kp = kps[p];
+        int rc = gp_make_kplan(&scheds[p], faithful, !(tuning && tuning->no_batch), !(tuning && tuning->no_pair), &kp);
+        if (rc != GP_OK) return rc;
+        if (tuning && tuning->no_truncate) std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
+        // heavy-task splitting only where a depth-2 loop exists
+        const bool split = !faithful && !kp.root_mode && kp.leaf >= 3 && split_req >= 0;
+        kp.split_min = split ? (split_req > 0 ? split_req : kp.pair ? GP_SPLIT_DEFAULT_PAIR : GP_SPLIT_DEFAULT) : INT_MAX;
+        any_split = any_split || split;
+        any_batched = any_batched || kp.batched;
+        any_k = any_k || kp.k;
+        if (kp.pair) hash_slots = std::max(hash_slots, GP_HASH_SLOTS_PAIR);
+        if (kp.batched) {
+            hash_slots = std::max(hash_slots, GP_HASH_SLOTS_BATCH);
+            bc_rows = std::max(bc_rows, kp.nblocks);
+            nbslots = std::max(nbslots, kp.nbslots);
+        }
+        max_slots = std::max(max_slots, kp.nslots);
     }
     HIP_TRY(hipSetDevice(g->device));
-    int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 20;
+    const int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 20;
     int64_t waves = (int64_t)g->num_cu * wpc;
-    const uint64_t all_tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
-    uint64_t t_begin = 0, t_end = all_tasks;
-    if (tuning) {
-        if (tuning->task_begin < 0 || (tuning->task_end > 0 && tuning->task_end < tuning->task_begin))
-            return gp_fail(GP_EINVAL, "bad task range");
-        t_begin = std::min<uint64_t>((uint64_t)tuning->task_begin, all_tasks);
-        if (tuning->task_end > 0) t_end = std::min<uint64_t>((uint64_t)tuning->task_end, all_tasks);
+    // work items: each plan's task range, concatenated
+    std::vector<uint64_t> task_off((size_t)nplans + 1, 0), task_first((size_t)nplans, 0), task_cnt((size_t)nplans, 0);
+    for (int p = 0; p < nplans; ++p) {
+        const uint64_t all_tasks = kps[p].root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+        uint64_t t_begin = 0, t_end = all_tasks;
+        if (tuning && (tuning->task_begin != 0 || tuning->task_end > 0)) {
+            if (nplans != 1) return gp_fail(GP_EINVAL, "task ranges apply to a single schedule");
+            if (tuning->task_begin < 0 || (tuning->task_end > 0 && tuning->task_end < tuning->task_begin))
+                return gp_fail(GP_EINVAL, "bad task range");
+            t_begin = std::min<uint64_t>((uint64_t)tuning->task_begin, all_tasks);
+            if (tuning->task_end > 0) t_end = std::min<uint64_t>((uint64_t)tuning->task_end, all_tasks);
+        }
+        task_first[p] = t_begin;
+        task_cnt[p] = t_end > t_begin ? t_end - t_begin : 0;
     }
-    const uint64_t n_tasks = t_end > t_begin ? t_end - t_begin : 0;
+    uint64_t real_tasks = 0;
+    for (int p = 0; p < nplans; ++p) real_tasks += task_cnt[p];
     // tasks per work fetch: 16, fewer when there are under 32 chunks per wave
     // (small graphs: the last chunks then decide the kernel's length)
-    int chunk = tuning && tuning->chunk > 0 ? tuning->chunk
-                                            : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, n_tasks / (uint64_t)(waves * 32)));
+    const int chunk = tuning && tuning->chunk > 0
+                          ? tuning->chunk
+                          : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, real_tasks / (uint64_t)(waves * 32)));
+    // each plan's items start on a chunk boundary, so no fetch spans two plans
+    for (int p = 0; p < nplans; ++p)
+        task_off[p + 1] = task_off[p] + (task_cnt[p] + (uint64_t)chunk - 1) / (uint64_t)chunk * (uint64_t)chunk;
+    const uint64_t n_tasks = task_off[nplans];
     if (waves < 1) waves = 1;
     if ((uint64_t)waves > n_tasks / (uint64_t)chunk + 1) waves = (int64_t)(n_tasks / (uint64_t)chunk + 1);
-    // scratch: per wave, one slot per materialised prefix (a subset of one
+    // scratch per wave: one slot per materialised prefix (a subset of one
     // neighbour list of a vertex other than the root, so at most the
-    // second-largest degree), plus two IEP temporaries in faithful mode
+    // second-largest degree), two IEP temporaries in faithful mode, and the
+    // batched level's block sets (each bounded by a neighbour list)
     const int64_t slot_stride = ((int64_t)std::max(g->max_deg2, 1) + 63) / 64 * 64;
     const int64_t deg_stride = ((int64_t)std::max(g->max_deg, 1) + 63) / 64 * 64;
-    const int64_t tmp_stride = faithful && kp.k ? deg_stride : 0;
-    // batched leaf level: intersections of a block's source sets (plus one
-    // spare for three or more sources); each is bounded by a neighbour list
-    const int64_t bslot_stride = kp.batched && kp.nbslots ? deg_stride : 0;
-    const int64_t wave_stride = kp.nslots * slot_stride + 2 * tmp_stride + (kp.nbslots + 1) * bslot_stride;
+    const int64_t tmp_stride = faithful && any_k ? deg_stride : 0;
+    const int64_t bslot_stride = any_batched && nbslots ? deg_stride : 0;
+    const int64_t wave_stride = max_slots * slot_stride + 2 * tmp_stride + (nbslots + 1) * bslot_stride;
     const size_t scratch_bytes = (size_t)std::max<int64_t>(waves * wave_stride, 64) * sizeof(int32_t);
     // LDS hash pool: tuning->lds_stage elements staged at load factor <= 1/2
-    int hash_slots = kp.pair ? GP_HASH_SLOTS_PAIR : kp.batched ? GP_HASH_SLOTS_BATCH : 0;
-    if (tuning && tuning->lds_stage > 0 && (kp.pair || kp.batched)) {
+    if (tuning && tuning->lds_stage > 0 && hash_slots > 0) {
         hash_slots = 64;
         while (hash_slots < 2 * tuning->lds_stage && hash_slots < GP_HASH_SLOTS_MAX) hash_slots <<= 1;
     }
     int hash_lg = 0;
     while ((1 << hash_lg) < hash_slots) ++hash_lg;
-    const size_t lds_bytes = sizeof(int32_t) * ((size_t)hash_slots + (kp.batched ? (size_t)kp.nblocks * WAVE : 0));
+    const size_t lds_bytes = sizeof(int32_t) * ((size_t)hash_slots + (size_t)bc_rows * WAVE);
 
     CallCtx* c = nullptr;
-    rc = acquire_ctx(g, scratch_bytes, &c);
+    int rc = acquire_ctx(g, scratch_bytes, &c);
     if (rc != GP_OK) return rc;
-    if (c->clocks_cap < (size_t)waves) {
-        if (c->d_clocks) (void)hipFree(c->d_clocks);
-        c->d_clocks = nullptr;
-        c->clocks_cap = 0;
+    const size_t plans_bytes = sizeof(KPlan) * (size_t)nplans;
+    const size_t meta_bytes = plans_bytes + sizeof(uint64_t) * (3 * (size_t)nplans + 1);
+    const size_t n_cnt = 3 + 3 * (size_t)nplans;
+    hipError_t e = ensure((void**)&c->d_meta, &c->meta_cap, meta_bytes);
+    if (e == hipSuccess) e = ensure((void**)&c->d_cnt, &c->cnt_cap, n_cnt * sizeof(unsigned long long));
+    if (e == hipSuccess && c->clocks_cap < (size_t)waves) {
         // start and finish stamps per wave, for each of the two launches
-        hipError_t ce = hipMalloc((void**)&c->d_clocks, sizeof(unsigned long long) * 4 * (size_t)waves);
-        if (ce != hipSuccess) {
-            destroy_ctx(c);
-            return gp_fail(GP_ENOMEM, std::string("clock buffer: ") + hipGetErrorString(ce));
-        }
-        c->clocks_cap = (size_t)waves;
+        size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
+        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves);
+        c->clocks_cap = e == hipSuccess ? (size_t)waves : 0;
     }
+    if (e != hipSuccess) {
+        destroy_ctx(c);
+        return gp_fail(GP_ENOMEM, std::string("call buffers: ") + hipGetErrorString(e));
+    }
+    std::vector<char> meta(meta_bytes);
+    std::memcpy(meta.data(), kps.data(), plans_bytes);
+    std::memcpy(meta.data() + plans_bytes, task_off.data(), sizeof(uint64_t) * ((size_t)nplans + 1));
+    std::memcpy(meta.data() + plans_bytes + sizeof(uint64_t) * ((size_t)nplans + 1), task_first.data(),
+                sizeof(uint64_t) * (size_t)nplans);
+    std::memcpy(meta.data() + plans_bytes + sizeof(uint64_t) * (2 * (size_t)nplans + 1), task_cnt.data(),
+                sizeof(uint64_t) * (size_t)nplans);
     KArgs a;
-    a.plan = c->d_plan;
+    a.plans = reinterpret_cast<const KPlan*>(c->d_meta);
+    a.nplans = nplans;
+    a.task_off = reinterpret_cast<const uint64_t*>(c->d_meta + plans_bytes);
+    a.task_first = reinterpret_cast<const uint64_t*>(c->d_meta + plans_bytes + sizeof(uint64_t) * ((size_t)nplans + 1));
+    a.task_cnt = reinterpret_cast<const uint64_t*>(c->d_meta + plans_bytes + sizeof(uint64_t) * (2 * (size_t)nplans + 1));
     a.vertex = g->d_vertex;
     a.edge = g->d_edge;
     a.src = g->d_src;
     a.v_cnt = g->v_cnt;
     a.chunk = chunk;
     a.n_tasks = n_tasks;
-    a.task_base = t_begin;
     a.counter = c->d_cnt;
+    a.counter2 = c->d_cnt + 1;
+    a.heavy_cnt = c->d_cnt + 2;
+    a.out = c->d_cnt + 3;
+    a.heavy = c->d_heavy;
     a.scratch = c->d_scratch;
     a.slot_stride = slot_stride;
     a.wave_stride = wave_stride;
@@ -1198,30 +1305,23 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     a.bslot_stride = bslot_stride;
     a.hash_slots = hash_slots;
     a.hash_lg = hash_lg;
-    a.out = c->d_cnt + 1;
     a.clocks = c->d_clocks;
-    // heavy-task splitting: only where a depth-2 loop exists
-    const int split_req = tuning ? tuning->split_min : 0;
-    const bool split = !faithful && !kp.root_mode && kp.leaf >= 3 && split_req >= 0;
-    a.split_min = split ? (split_req > 0 ? split_req : kp.pair ? GP_SPLIT_DEFAULT_PAIR : GP_SPLIT_DEFAULT) : INT_MAX;
-    a.heavy = c->d_heavy;
-    a.heavy_cnt = c->d_cnt + 5;
-    a.counter2 = c->d_cnt + 4;
-    unsigned long long host_out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<unsigned long long> host_out(n_cnt, 0);
     std::vector<unsigned long long> clocks(4 * (size_t)waves, 0);
-    hipError_t e = hipMemcpyAsync(c->d_plan, &kp, sizeof(KPlan), hipMemcpyHostToDevice, c->stream);
+    e = hipMemcpyAsync(c->d_meta, meta.data(), meta_bytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt, 8);
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, (int)n_cnt);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+    const bool parts_launch = any_split && n_tasks > 0;
     if (e == hipSuccess && n_tasks > 0) {
         if (faithful)
             hipLaunchKernelGGL((gp_count_kernel<true, false>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
         else
             hipLaunchKernelGGL((gp_count_kernel<false, false>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
         e = hipGetLastError();
-        if (e == hipSuccess && split) {
+        if (e == hipSuccess && parts_launch) {
             // the deferred tasks, in parts; the kernel reads how many there are
             KArgs a2 = a;
             a2.clocks = c->d_clocks + 2 * waves;
@@ -1230,9 +1330,10 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
         }
     }
     if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host_out.data(), c->d_cnt, n_cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && n_tasks > 0)
-        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * (split ? 4 : 2) * (size_t)waves,
+        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * (parts_launch ? 4 : 2) * (size_t)waves,
                            hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     float ms = 0;
@@ -1243,20 +1344,26 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
         return gp_fail((int)e, std::string("count kernel: ") + hipGetErrorString(e));
     }
     release_ctx(g, c);
-    const uint64_t sum = host_out[1];
-    if (raw) *raw = sum;
-    if (count) *count = (int64_t)sum / (sched->redundancy ? sched->redundancy : 1);
-    if (stats) {
-        stats->kernel_ms = ms;
+    uint64_t balg_total = 0, tasks_total = 0;
+    for (int p = 0; p < nplans; ++p) {
+        const uint64_t sum = host_out[3 + 3 * p];
+        if (raw) raw[p] = sum;
+        if (count) count[p] = (int64_t)sum / (scheds[p].redundancy ? scheds[p].redundancy : 1);
         // every root's neighbour list is looked up once by the reference;
         // those lookups are attributed to the range that starts at task 0
-        stats->balg_bytes = faithful ? host_out[2] + (t_begin == 0 && !kp.root_mode ? 8ull * (uint64_t)g->v_cnt : 0) +
-                                           (kp.root_mode ? 8ull * n_tasks : 0)
-                                     : 0;
-        stats->tasks = host_out[3];
-        stats->split_tasks = split ? std::min<uint64_t>(host_out[5], GP_HEAVY_CAP) : 0;
+        if (faithful)
+            balg_total += host_out[4 + 3 * p] +
+                          (kps[p].root_mode ? 8ull * task_cnt[p]
+                                            : task_first[p] == 0 ? 8ull * (uint64_t)g->v_cnt : 0);
+        tasks_total += host_out[5 + 3 * p];
+    }
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->balg_bytes = balg_total;
+        stats->tasks = tasks_total;
         stats->waves = (int32_t)waves;
         stats->kernel_variant = faithful ? 1 : 0;
+        stats->split_tasks = parts_launch ? std::min<uint64_t>(host_out[2], GP_HEAVY_CAP) : 0;
         stats->finish_p50_ms = stats->finish_p90_ms = stats->finish_max_ms = 0;
         if (n_tasks > 0) {
             int rate_khz = 0;
@@ -1268,7 +1375,7 @@ int run_count(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
             std::vector<double> fin((size_t)waves);
             for (int64_t w = 0; w < waves; ++w) {
                 unsigned long long f = clocks[2 * w + 1];
-                if (split) f = std::max(f, clocks[2 * waves + 2 * w + 1]);
+                if (parts_launch) f = std::max(f, clocks[2 * waves + 2 * w + 1]);
                 fin[w] = (double)(f - t0) / rate_khz;
             }
             std::sort(fin.begin(), fin.end());
@@ -1350,7 +1457,13 @@ extern "C" int gp_graph_bytes(const gp_graph* g, uint64_t* bytes) {
 extern "C" int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw_sum,
                         int64_t* count, gp_stats* stats) {
     if (!g || !sched) return gp_fail(GP_EINVAL, "null argument");
-    return run_count(const_cast<gp_graph*>(g), sched, tuning, raw_sum, count, stats);
+    return run_plans(const_cast<gp_graph*>(g), sched, 1, tuning, raw_sum, count, stats);
+}
+
+extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
+                              const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats) {
+    if (!g || !scheds) return gp_fail(GP_EINVAL, "null argument");
+    return run_plans(const_cast<gp_graph*>(g), scheds, n_scheds, tuning, raw_sums, counts, stats);
 }
 
 extern "C" int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks) {
@@ -1372,7 +1485,7 @@ extern "C" int gp_tc(const gp_graph* g, uint64_t* triangles, gp_stats* stats) {
     int rc = s.flatten(&flat);
     if (rc != GP_OK) return gp_fail(rc, "triangle schedule");
     uint64_t raw = 0;
-    rc = run_count(const_cast<gp_graph*>(g), &flat, nullptr, &raw, nullptr, stats);
+    rc = run_plans(const_cast<gp_graph*>(g), &flat, 1, nullptr, &raw, nullptr, stats);
     if (rc != GP_OK) return rc;
     *triangles = raw;
     return GP_OK;

@@ -53,6 +53,9 @@ struct KPlan {
     // with G(a) = g ∩ N(a), X = [0, bound(leaf)) minus placed vertices.
     int32_t pair;
     int32_t pair_g;   // prefix g, -1 = every vertex
+    // set by the launcher: a task whose depth-2 loop has more candidates is
+    // deferred and run in parts (INT32_MAX: never)
+    int32_t split_min;
 };
 
 // Returns GP_OK or an error code (with gp_fail message) for unsupported input.

@@ -16,6 +16,7 @@
 #include "../hip/kplan.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -147,6 +148,7 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool 
 
     kp->fused = 0;
     kp->fused_prefix = -1;
+    kp->split_min = INT32_MAX; // the launcher decides
 
     // ---- scratch slots ----
     // in batched mode the sets of depth leaf-1 are never built: the leaf

@@ -141,6 +141,17 @@ GP_API int gp_graph_bytes(const gp_graph* g, uint64_t* bytes);
 GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
                     uint64_t* raw_sum, int64_t* count, gp_stats* stats);
 
+/* Count n_scheds schedules over one uploaded graph in one persistent launch
+ * (plus one launch for the parts of deferred heavy tasks): the work items of
+ * every schedule share one work counter, so schedules of uneven cost balance
+ * across the device.  raw_sums[i] / counts[i] equal what gp_count returns
+ * for scheds[i]; stats (optional) covers the whole batch.  tuning applies to
+ * every schedule; a task range is rejected (GP_EINVAL) when n_scheds > 1.
+ * At most 256 schedules per call.  No reference counterpart: it serves the
+ * many-pattern drivers (MotifGenerator output, all_schedule_test.cpp). */
+GP_API int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
+                          const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats);
+
 /* Number of tasks gp_count splits the schedule's count into (see
  * gp_tuning.task_begin). */
 GP_API int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks);

@@ -316,3 +316,75 @@ def test_bad_schedule_is_rejected():
     s2.loop_set[2] = 27
     with pytest.raises(gp.GPError):
         dev.count(s2)
+
+
+def _motif_schedules(host, sizes, tri, settings=(1, 1, 1)):
+    out = []
+    for k in sizes:
+        for adj in gp.motifs(k):
+            s, valid = gp.build_schedule(adj, *settings, host.v_cnt, host.e_cnt, tri)
+            if valid:
+                out.append(s)
+    return out
+
+
+def test_batch_equals_single_counts(rmat13_path):
+    # every connected 3-, 4- and 5-vertex pattern (2 + 6 + 21) in one launch
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    scheds = _motif_schedules(host, (3, 4, 5), tri)
+    assert len(scheds) == 29
+    res, st = dev.count_batch(scheds)
+    single_ms = 0.0
+    for s, r in zip(scheds, res):
+        one = dev.count(s)
+        single_ms += one.stats.kernel_ms
+        assert (r.count, r.raw) == (one.count, one.raw)
+    print(f"29 patterns: batch {st.kernel_ms:.2f} ms, one at a time {single_ms:.2f} ms, "
+          f"{st.split_tasks} split tasks", flush=True)
+    # heavy-task deferral across plans, and the per-candidate path
+    tun = gp.Tuning()
+    tun.split_min = 1
+    res2, st2 = dev.count_batch(scheds, tun)
+    assert st2.split_tasks > 0
+    assert [r.raw for r in res2] == [r.raw for r in res]
+    tun.no_batch = 1
+    assert [r.raw for r in dev.count_batch(scheds, tun)[0]] == [r.raw for r in res]
+
+
+def test_batch_matches_oracle_and_balg():
+    host = _random_graph(90, 0.1, 5, hubs=3)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    scheds = _motif_schedules(host, (4, 5, 6), tri)
+    assert len(scheds) == 6 + 21 + 112
+    res, _ = dev.count_batch(scheds)
+    for s, r in zip(scheds, res):
+        want = ob.pattern_matching(host, s, threads=4)
+        assert (r.count, r.raw) == (want["count"], want["raw"])
+    # faithful mode: the batch's B_alg is the sum of the single counts'
+    tun = gp.Tuning()
+    tun.collect_balg = 1
+    few = scheds[:12]
+    _, st = dev.count_batch(few, tun)
+    assert st.balg_bytes == sum(dev.count(s, tun).stats.balg_bytes for s in few)
+    # the same schedule twice, and a batch of one
+    s = scheds[3]
+    twice, _ = dev.count_batch([s, s])
+    assert twice[0].raw == twice[1].raw == dev.count(s).raw
+    assert dev.count_batch([s])[0][0].raw == twice[0].raw
+
+
+def test_batch_rejects_bad_input(rmat13_path):
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    s = _sched(host, gp.PATTERNS["house"], 1, 1, 1)
+    tun = gp.Tuning()
+    tun.task_begin, tun.task_end = 0, 100
+    with pytest.raises(gp.GPError):
+        dev.count_batch([s, s], tun)
+    with pytest.raises(gp.GPError):
+        dev.count_batch([])
+    with pytest.raises(gp.GPError):
+        dev.count_batch([s] * 257)
