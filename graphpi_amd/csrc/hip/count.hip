@@ -293,8 +293,11 @@ __device__ __forceinline__ uint32_t staged_has_n(const int32_t* tab, int lg, con
     return hits;
 }
 
-template <bool FAITHFUL, bool PARTS>
-__global__ void __launch_bounds__(WAVE, 5) gp_count_kernel(KArgs a) {
+// FAITHFUL: replay the reference's intersections and tally B_alg.
+// PARTS: run the parts of deferred heavy tasks instead of the task list.
+// MULTI: several plans share the launch (one plan when false).
+template <bool FAITHFUL, bool PARTS, bool MULTI>
+__global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
     __shared__ int32_t s_len[GP_MAX_PREFIX];
     __shared__ int32_t s_emb[GP_MAX_SIZE];
@@ -918,10 +921,12 @@ __global__ void __launch_bounds__(WAVE, 5) gp_count_kernel(KArgs a) {
                 if (base >= n_items) break;
                 // the plan owning this chunk; a chunk never spans two plans
                 int p = 0;
-                while (p + 1 < a.nplans && a.task_off[p + 1] <= base) ++p;
-                if (p != cur_plan) {
-                    flush();
-                    set_plan(p);
+                if (MULTI) {
+                    while (p + 1 < a.nplans && a.task_off[p + 1] <= base) ++p;
+                    if (p != cur_plan) {
+                        flush();
+                        set_plan(p);
+                    }
                 }
                 cur = base;
                 cend = min((uint64_t)base + (uint64_t)a.chunk, (uint64_t)(a.task_off[p] + a.task_cnt[p]));
@@ -940,10 +945,12 @@ __global__ void __launch_bounds__(WAVE, 5) gp_count_kernel(KArgs a) {
             const int32_t part = (int32_t)(w % GP_SPLIT_PARTS);
             const int32_t parts = max(2, min(GP_SPLIT_PARTS, (int32_t)(e & 0xFFFFFF) / GP_SPLIT_GRAIN));
             if (part >= parts) continue;
-            const int p = (int)(e >> 56);
-            if (p != cur_plan) {
-                flush();
-                set_plan(p);
+            if (MULTI) {
+                const int p = (int)(e >> 56);
+                if (p != cur_plan) {
+                    flush();
+                    set_plan(p);
+                }
             }
             dstart = part;
             dstride = parts;
@@ -1207,7 +1214,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         max_slots = std::max(max_slots, kp.nslots);
     }
     HIP_TRY(hipSetDevice(g->device));
-    const int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 20;
+    const int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 24;
     int64_t waves = (int64_t)g->num_cu * wpc;
     // work items: each plan's task range, concatenated
     std::vector<uint64_t> task_off((size_t)nplans + 1, 0), task_first((size_t)nplans, 0), task_cnt((size_t)nplans, 0);
@@ -1316,16 +1323,28 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
     const bool parts_launch = any_split && n_tasks > 0;
     if (e == hipSuccess && n_tasks > 0) {
-        if (faithful)
-            hipLaunchKernelGGL((gp_count_kernel<true, false>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
-        else
-            hipLaunchKernelGGL((gp_count_kernel<false, false>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a);
+        const dim3 grid((unsigned)waves), block(WAVE);
+        const bool multi = nplans > 1;
+        if (faithful) {
+            if (multi)
+                hipLaunchKernelGGL((gp_count_kernel<true, false, true>), grid, block, lds_bytes, c->stream, a);
+            else
+                hipLaunchKernelGGL((gp_count_kernel<true, false, false>), grid, block, lds_bytes, c->stream, a);
+        } else {
+            if (multi)
+                hipLaunchKernelGGL((gp_count_kernel<false, false, true>), grid, block, lds_bytes, c->stream, a);
+            else
+                hipLaunchKernelGGL((gp_count_kernel<false, false, false>), grid, block, lds_bytes, c->stream, a);
+        }
         e = hipGetLastError();
         if (e == hipSuccess && parts_launch) {
             // the deferred tasks, in parts; the kernel reads how many there are
             KArgs a2 = a;
             a2.clocks = c->d_clocks + 2 * waves;
-            hipLaunchKernelGGL((gp_count_kernel<false, true>), dim3((unsigned)waves), dim3(WAVE), lds_bytes, c->stream, a2);
+            if (multi)
+                hipLaunchKernelGGL((gp_count_kernel<false, true, true>), grid, block, lds_bytes, c->stream, a2);
+            else
+                hipLaunchKernelGGL((gp_count_kernel<false, true, false>), grid, block, lds_bytes, c->stream, a2);
             e = hipGetLastError();
         }
     }

@@ -85,7 +85,7 @@ typedef struct gp_sched_flat {
 
 /* Counting options.  Zero-initialise and set what you need. */
 typedef struct gp_tuning {
-    int32_t waves_per_cu;   /* persistent waves per CU; 0 = default (20) */
+    int32_t waves_per_cu;   /* persistent waves per CU; 0 = default (24) */
     int32_t chunk;          /* tasks per work fetch; 0 = 16, or fewer when there are under
                                32 fetches per wave */
     int32_t lds_stage;      /* set elements the LDS hash pool holds (pool = the power of two
