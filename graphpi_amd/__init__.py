@@ -139,6 +139,8 @@ _SIGS = {
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(Stats)]),
     "gp_count_tasks": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(ctypes.c_uint64)]),
+    "gp_count_footprint": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
+                                          ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64)]),
     "gp_count_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
                                       ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]),
@@ -393,6 +395,14 @@ class DeviceGraph:
         _check(lib().gp_count_batch(self._h, arr, n, ctypes.byref(tuning) if tuning else None, raws, cnts,
                                     ctypes.byref(st)))
         return [CountResult(cnts[i], raws[i], None) for i in range(n)], st
+
+    def footprint(self, scheds, tuning=None):
+        """Device bytes (besides the graph) a count_batch of scheds would use."""
+        n = len(scheds)
+        arr = (SchedFlat * n)(*scheds)
+        b = ctypes.c_uint64()
+        _check(lib().gp_count_footprint(self._h, arr, n, ctypes.byref(tuning) if tuning else None, ctypes.byref(b)))
+        return b.value
 
     def count_tasks(self, sched):
         t = ctypes.c_uint64()

@@ -1182,11 +1182,28 @@ hipError_t ensure(void** p, size_t* cap, size_t bytes) {
     return e;
 }
 
-// Count nplans schedules over one graph in one persistent launch (plus the
-// launch of deferred parts).  Results are per schedule; a task range in the
-// tuning is only accepted for a single schedule.
-int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, uint64_t* raw,
-              int64_t* count, gp_stats* stats) {
+// The launch of one counting call: kernel plans, work layout and resources.
+struct Launch {
+    bool faithful = false, any_split = false;
+    std::vector<KPlan> kps;
+    std::vector<uint64_t> task_off, task_first, task_cnt;
+    uint64_t n_tasks = 0;
+    int64_t waves = 0;
+    int chunk = 1;
+    int64_t slot_stride = 0, tmp_stride = 0, bslot_stride = 0, wave_stride = 0;
+    size_t scratch_bytes = 0;
+    int hash_slots = 0, hash_lg = 0;
+    size_t lds_bytes = 0;
+    // device memory the call needs besides the graph: scratch, plans and
+    // work layout, counters, clock stamps and the deferred-task list
+    size_t device_bytes() const {
+        const size_t n = kps.size();
+        return scratch_bytes + sizeof(KPlan) * n + sizeof(uint64_t) * (3 * n + 1) +
+               sizeof(unsigned long long) * (3 + 3 * n + 4 * (size_t)waves) + sizeof(unsigned long long) * GP_HEAVY_CAP;
+    }
+};
+
+int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, Launch* out) {
     if (nplans < 1 || nplans > GP_MAX_BATCH) return gp_fail(GP_ERANGE, "schedule count out of range");
     const bool faithful = tuning && tuning->collect_balg;
     std::vector<KPlan> kps((size_t)nplans);
@@ -1213,7 +1230,6 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         }
         max_slots = std::max(max_slots, kp.nslots);
     }
-    HIP_TRY(hipSetDevice(g->device));
     const int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 24;
     int64_t waves = (int64_t)g->num_cu * wpc;
     // work items: each plan's task range, concatenated
@@ -1263,6 +1279,48 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     while ((1 << hash_lg) < hash_slots) ++hash_lg;
     const size_t lds_bytes = sizeof(int32_t) * ((size_t)hash_slots + (size_t)bc_rows * WAVE);
 
+    out->faithful = faithful;
+    out->any_split = any_split;
+    out->kps = std::move(kps);
+    out->task_off = std::move(task_off);
+    out->task_first = std::move(task_first);
+    out->task_cnt = std::move(task_cnt);
+    out->n_tasks = n_tasks;
+    out->waves = waves;
+    out->chunk = chunk;
+    out->slot_stride = slot_stride;
+    out->tmp_stride = tmp_stride;
+    out->bslot_stride = bslot_stride;
+    out->wave_stride = wave_stride;
+    out->scratch_bytes = scratch_bytes;
+    out->hash_slots = hash_slots;
+    out->hash_lg = hash_lg;
+    out->lds_bytes = lds_bytes;
+    return GP_OK;
+}
+
+// Count nplans schedules over one graph in one persistent launch (plus the
+// launch of deferred parts).  Results are per schedule; a task range in the
+// tuning is only accepted for a single schedule.
+int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, uint64_t* raw,
+              int64_t* count, gp_stats* stats) {
+    Launch ln;
+    {
+        int rc = prepare(g, scheds, nplans, tuning, &ln);
+        if (rc != GP_OK) return rc;
+    }
+    HIP_TRY(hipSetDevice(g->device));
+    const bool faithful = ln.faithful, any_split = ln.any_split;
+    const std::vector<KPlan>& kps = ln.kps;
+    const std::vector<uint64_t>&task_off = ln.task_off, &task_first = ln.task_first, &task_cnt = ln.task_cnt;
+    const uint64_t n_tasks = ln.n_tasks;
+    const int64_t waves = ln.waves;
+    const int chunk = ln.chunk;
+    const int64_t slot_stride = ln.slot_stride, tmp_stride = ln.tmp_stride, bslot_stride = ln.bslot_stride,
+                  wave_stride = ln.wave_stride;
+    const size_t scratch_bytes = ln.scratch_bytes;
+    const int hash_slots = ln.hash_slots, hash_lg = ln.hash_lg;
+    const size_t lds_bytes = ln.lds_bytes;
     CallCtx* c = nullptr;
     int rc = acquire_ctx(g, scratch_bytes, &c);
     if (rc != GP_OK) return rc;
@@ -1483,6 +1541,16 @@ extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, in
                               const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats) {
     if (!g || !scheds) return gp_fail(GP_EINVAL, "null argument");
     return run_plans(const_cast<gp_graph*>(g), scheds, n_scheds, tuning, raw_sums, counts, stats);
+}
+
+extern "C" int gp_count_footprint(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
+                                  const gp_tuning* tuning, uint64_t* bytes) {
+    if (!g || !scheds || !bytes) return gp_fail(GP_EINVAL, "null argument");
+    Launch ln;
+    int rc = prepare(g, scheds, n_scheds, tuning, &ln);
+    if (rc != GP_OK) return rc;
+    *bytes = ln.device_bytes();
+    return GP_OK;
 }
 
 extern "C" int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks) {

@@ -1,0 +1,240 @@
+"""Serving loop: resident graphs, counting queries admitted as they arrive.
+
+A CountServer keeps several graphs resident in HBM and answers counting
+requests (a pattern, its planner settings and a graph name).  Worker slots
+take requests continuously: a slot that frees up immediately gathers the
+requests waiting for one graph (oldest first), up to a batch size and within
+the device-memory budget (each batch's footprint comes from
+gp_count_footprint), and counts them in one gp_count_batch launch.  Slots run
+on their own threads and HIP streams, so a batch that arrives while others are
+on the GPU starts as soon as a slot and memory are free and its kernels overlap
+with theirs.  Every result is the exact count: batching and concurrency change
+when a request runs, never what it returns.
+
+Schedules are built on the host (the planner) and cached per (graph, pattern,
+settings), as a server would cache them per canonical pattern.
+"""
+import collections
+import threading
+import time
+
+import numpy as np
+
+from . import DeviceGraph, build_schedule
+
+
+class Request:
+    __slots__ = ("rid", "graph", "adj", "settings", "t_arrival", "t_start", "t_done", "count", "raw", "error",
+                 "sched", "_event")
+
+    def __init__(self, rid, graph, adj, settings=(1, 1, 1), t_arrival=None):
+        self.rid = rid
+        self.graph = graph
+        self.adj = np.asarray(adj, dtype=np.int32)
+        self.settings = tuple(settings)
+        self.t_arrival = time.perf_counter() if t_arrival is None else t_arrival
+        self.t_start = self.t_done = None
+        self.count = self.raw = None
+        self.error = None
+        self.sched = None
+        self._event = threading.Event()
+
+    def wait(self, timeout=None):
+        if not self._event.wait(timeout):
+            raise TimeoutError(f"request {self.rid} not finished")
+        if self.error is not None:
+            raise self.error
+        return self.count
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+class CountServer:
+    """graphs: {name: HostGraph}.  slots: concurrent batches on the device.
+    max_batch: schedules per launch.  mem_budget: device bytes the in-flight
+    batches may use besides the graphs (a batch larger than the budget still
+    runs, alone)."""
+
+    def __init__(self, graphs, slots=4, max_batch=32, mem_budget=32 << 30, tuning=None, device_factory=None):
+        make = device_factory or DeviceGraph
+        self._dev = {}
+        self._host = {}
+        self._tri = {}
+        for name, host in graphs.items():
+            d = make(host)
+            self._dev[name] = d
+            self._host[name] = host
+            self._tri[name] = d.triangle_counting()
+        self._tuning = tuning
+        self._max_batch = max_batch
+        self._budget = mem_budget
+        self._in_flight = 0
+        self._pending = collections.deque()
+        self._cv = threading.Condition()
+        self._closed = False
+        self._sched_cache = {}
+        self.batches = []  # (graph, size, footprint, seconds) per launch
+        self._workers = [threading.Thread(target=self._run, name=f"count-slot-{i}", daemon=True)
+                         for i in range(slots)]
+        for w in self._workers:
+            w.start()
+
+    # -- client side --
+    def submit(self, req):
+        """Plan the request (on the caller's thread, cached) and queue it.  A
+        pattern the planner rejects fails at once without reaching a slot."""
+        if req.graph not in self._dev:
+            raise KeyError(f"unknown graph {req.graph}")
+        try:
+            req.sched = self.schedule_for(req.graph, req.adj, req.settings)
+        except Exception as e:
+            req.error = e
+            req.t_done = time.perf_counter()
+            req._event.set()
+            return req
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("server closed")
+            self._pending.append(req)
+            self._cv.notify()
+        return req
+
+    def close(self):
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        for w in self._workers:
+            w.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- scheduling --
+    def schedule_for(self, graph, adj, settings):
+        key = (graph, adj.tobytes(), adj.shape[0], settings)
+        with self._cv:
+            s = self._sched_cache.get(key)
+        if s is None:
+            h = self._host[graph]
+            pm, rt, iep = settings
+            s, valid = build_schedule(adj, pm, rt, iep, h.v_cnt, h.e_cnt, self._tri[graph])
+            if not valid:
+                raise ValueError("pattern has a disconnected prefix under these settings")
+            with self._cv:
+                self._sched_cache[key] = s
+        return s
+
+    def _take_batch(self):
+        """Called with the lock held: the requests of the oldest request's graph,
+        up to max_batch, within the memory budget.  None when nothing fits."""
+        if not self._pending:
+            return None
+        graph = self._pending[0].graph
+        batch, rest = [], collections.deque()
+        while self._pending:
+            r = self._pending.popleft()
+            if r.graph == graph and len(batch) < self._max_batch:
+                batch.append(r)
+            else:
+                rest.append(r)
+        self._pending = rest
+        ok = batch
+        scheds = [r.sched for r in ok]
+        dev = self._dev[graph]
+        fp = dev.footprint(scheds, self._tuning)
+        # admit the longest prefix that fits the free budget (at least one
+        # request when nothing else is in flight)
+        while ok and self._in_flight + fp > self._budget and (self._in_flight > 0 or len(ok) > 1):
+            self._pending.appendleft(ok.pop())
+            scheds.pop()
+            fp = dev.footprint(scheds, self._tuning) if scheds else 0
+        if not ok:
+            return None
+        self._in_flight += fp
+        return (graph, ok, scheds, fp)
+
+    def _run(self):
+        while True:
+            with self._cv:
+                item = None
+                while item is None:
+                    if self._closed and not self._pending:
+                        return
+                    item = self._take_batch()
+                    if item is None:
+                        self._cv.wait(0.05)
+            graph, reqs, scheds, fp = item
+            t0 = time.perf_counter()
+            for r in reqs:
+                r.t_start = t0
+            try:
+                res, _ = self._dev[graph].count_batch(scheds, self._tuning)
+                err = None
+            except Exception as e:
+                res, err = None, e
+            t1 = time.perf_counter()
+            with self._cv:
+                self._in_flight -= fp
+                self.batches.append((graph, len(reqs), fp, t1 - t0))
+                self._cv.notify_all()
+            for i, r in enumerate(reqs):
+                r.t_done = t1
+                if err is None:
+                    r.count, r.raw = res[i].count, res[i].raw
+                else:
+                    r.error = err
+                r._event.set()
+
+
+# ---------------------------------------------------------------------------
+# synthetic load
+# ---------------------------------------------------------------------------
+
+def random_pattern(rng, k, p_extra=0.3):
+    """A connected k-vertex pattern: a random spanning tree (each vertex i > 0
+    joins a uniformly chosen earlier vertex, then the labels are shuffled)
+    plus every other pair independently with probability p_extra."""
+    adj = np.zeros((k, k), dtype=np.int32)
+    for i in range(1, k):
+        j = int(rng.integers(0, i))
+        adj[i, j] = adj[j, i] = 1
+    for i in range(k):
+        for j in range(i + 1, k):
+            if not adj[i, j] and rng.random() < p_extra:
+                adj[i, j] = adj[j, i] = 1
+    perm = rng.permutation(k)
+    return adj[np.ix_(perm, perm)]
+
+
+def poisson_arrivals(rng, rate, n):
+    """n arrival offsets (seconds) of a Poisson process with the given rate."""
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def run_load(server, requests, offsets):
+    """Submit requests at their arrival offsets (seconds from now) and wait for
+    all of them.  Returns the wall time from the first arrival to the last
+    completion."""
+    t0 = time.perf_counter()
+    for r, off in zip(requests, offsets):
+        now = time.perf_counter() - t0
+        if off > now:
+            time.sleep(off - now)
+        r.t_arrival = time.perf_counter()
+        server.submit(r)
+    for r in requests:
+        r._event.wait()
+    return max(r.t_done for r in requests) - t0

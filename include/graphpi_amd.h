@@ -152,6 +152,14 @@ GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuni
 GP_API int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
                           const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats);
 
+/* Device memory (bytes, besides the uploaded graph) that gp_count_batch with
+ * the same arguments would use: per-wave scratch, plans, counters, stamps
+ * and the deferred-task list.  Lets a server admit requests by footprint.
+ * No reference counterpart (the reference allocates per thread on the host,
+ * src/vertex_set.cpp:10-48). */
+GP_API int gp_count_footprint(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
+                              const gp_tuning* tuning, uint64_t* bytes);
+
 /* Number of tasks gp_count splits the schedule's count into (see
  * gp_tuning.task_begin). */
 GP_API int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks);

@@ -1,0 +1,102 @@
+"""The serving loop's scheduling logic on the CPU, with a stand-in device
+(the planner is the real host planner; counting is replaced by a
+deterministic function of the schedule), and the synthetic load generators."""
+import threading
+import time
+
+import numpy as np
+
+import graphpi_amd as gp
+from graphpi_amd.serve import CountServer, Request, percentile, poisson_arrivals, random_pattern, run_load
+
+
+class FakeDevice:
+    """count_batch returns a value derived from each schedule alone, takes
+    time, and records how much footprint was in flight at once."""
+    lock = threading.Lock()
+
+    def __init__(self, host, log):
+        self.host = host
+        self.log = log
+
+    def triangle_counting(self):
+        return 1000
+
+    def footprint(self, scheds, tuning=None):
+        return 100 * len(scheds)
+
+    def count_batch(self, scheds, tuning=None):
+        with FakeDevice.lock:
+            self.log["in_flight"] += self.footprint(scheds)
+            self.log["peak"] = max(self.log["peak"], self.log["in_flight"])
+            self.log["sizes"].append(len(scheds))
+        time.sleep(0.002 * len(scheds))
+        out = [gp.CountResult(self.value(s), self.value(s), None) for s in scheds]
+        with FakeDevice.lock:
+            self.log["in_flight"] -= self.footprint(scheds)
+        return out, gp.Stats()
+
+    @staticmethod
+    def value(s):
+        d = s.to_dict()
+        return hash((tuple(d["adj"]), tuple(map(tuple, d["restrict_pair"])), d["size"])) & 0xFFFFFFFF
+
+
+def _server(log, **kw):
+    graphs = {"a": gp.HostGraph.complete(8), "b": gp.HostGraph.complete(9)}
+    return CountServer(graphs, device_factory=lambda h: FakeDevice(h, log), **kw)
+
+
+def test_random_patterns_connected_and_deterministic():
+    for k in range(3, 8):
+        a1 = random_pattern(np.random.default_rng(k), k)
+        a2 = random_pattern(np.random.default_rng(k), k)
+        assert (a1 == a2).all() and (a1 == a1.T).all() and not a1.diagonal().any()
+        seen, stack = {0}, [0]
+        while stack:
+            v = stack.pop()
+            for u in np.nonzero(a1[v])[0]:
+                if u not in seen:
+                    seen.add(int(u))
+                    stack.append(int(u))
+        assert len(seen) == k
+
+
+def test_poisson_arrivals_rate():
+    off = poisson_arrivals(np.random.default_rng(1), 200.0, 4000)
+    assert abs(len(off) / off[-1] - 200.0) < 20.0
+    assert (np.diff(off) > 0).all()
+
+
+def test_results_same_alone_and_in_the_mix():
+    log = {"in_flight": 0, "peak": 0, "sizes": []}
+    rng = np.random.default_rng(3)
+    reqs = [Request(i, "ab"[i % 2], random_pattern(rng, int(rng.integers(3, 7)))) for i in range(120)]
+    with _server(log, slots=3, max_batch=8, mem_budget=1500) as srv:
+        offs = poisson_arrivals(rng, 2000.0, len(reqs))
+        run_load(srv, reqs, offs)
+        for r in reqs:
+            alone = FakeDevice.value(srv.schedule_for(r.graph, r.adj, r.settings))
+            assert r.wait() == alone
+            assert r.latency >= 0
+    assert max(log["sizes"]) <= 8
+    assert log["peak"] <= 1500
+    assert len(log["sizes"]) < len(reqs)  # requests were batched
+    lat = [r.latency for r in reqs]
+    assert percentile(lat, 50) <= percentile(lat, 99)
+
+
+def test_bad_pattern_fails_alone():
+    log = {"in_flight": 0, "peak": 0, "sizes": []}
+    disconnected = np.zeros((4, 4), dtype=np.int32)
+    disconnected[0, 1] = disconnected[1, 0] = 1
+    disconnected[2, 3] = disconnected[3, 2] = 1
+    with _server(log, slots=1) as srv:
+        bad = srv.submit(Request(0, "a", disconnected))
+        good = srv.submit(Request(1, "a", gp.PATTERNS["triangle"]))
+        assert good.wait(10) is not None
+        try:
+            bad.wait(10)
+            assert False, "a disconnected pattern must fail"
+        except ValueError:
+            pass
