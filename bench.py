@@ -88,6 +88,88 @@ def cpu_baseline(host, scheds, names, budget_s=20.0):
             "sample": "; ".join(samples) + "; full time estimated as sample time x stride"}
 
 
+def batched_bench(gp, scale=10):
+    """Every connected 5- and 6-vertex pattern (21 + 112, MotifGenerator order)
+    over one resident R-MAT graph (edge factor 16, seed 4): one gp_count_batch
+    call against the same schedules counted one gp_count call at a time."""
+    host = gp.HostGraph.rmat(scale, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = dev.triangle_counting()
+    scheds = []
+    for k in (5, 6):
+        for adj in gp.motifs(k):
+            sch, valid = gp.build_schedule(adj, 1, 1, True, host.v_cnt, host.e_cnt, tri)
+            assert valid
+            scheds.append(sch)
+    dev.count_batch(scheds[:4])  # warm-up
+    t0 = time.perf_counter()
+    loop_ms, singles = 0.0, []
+    for sch in scheds:
+        r = dev.count(sch)
+        loop_ms += r.stats.kernel_ms
+        singles.append(r.raw)
+    loop_wall = time.perf_counter() - t0
+    log(f"batched: one at a time {loop_wall:.2f} s ({loop_ms:.0f} ms kernel)")
+    t0 = time.perf_counter()
+    res, st = dev.count_batch(scheds)
+    batch_wall = time.perf_counter() - t0
+    exact = all(r.raw == x for r, x in zip(res, singles))
+    log(f"batched: one launch {batch_wall:.2f} s ({st.kernel_ms:.0f} ms kernel), identical={exact}")
+    assert exact
+    n = len(scheds)
+    return {"workload": f"R-MAT scale {scale}, edge factor 16, seed 4; all {n} connected 5- and 6-vertex patterns",
+            "v_cnt": host.v_cnt, "undirected_edges": host.e_cnt // 2,
+            "patterns_per_s_batched": round(n / batch_wall, 3), "patterns_per_s_one_at_a_time": round(n / loop_wall, 3),
+            "batched_s": round(batch_wall, 4), "one_at_a_time_s": round(loop_wall, 4),
+            "speedup": round(loop_wall / batch_wall, 3), "exact": exact}
+
+
+def served_bench(gp, scales=(8, 9, 10, 11), rates=(50, 200, 800), n_req=120, seed=12):
+    """Counting requests against resident graphs (R-MAT edge factor 16, seeds
+    5.., one per scale), random connected 3-6-vertex patterns (spanning tree +
+    Bernoulli(0.3) extra edges), Poisson arrivals at each rate: latency p50/p99
+    and throughput of the serving loop, against the same requests served one at
+    a time by plain gp_count calls (a serial queue)."""
+    import numpy as np
+    from graphpi_amd.serve import CountServer, Request, percentile, poisson_arrivals, random_pattern, run_load
+    graphs = {f"s{sc}": gp.HostGraph.rmat(sc, 16, 0.57, 0.19, 0.19, 5 + i) for i, sc in enumerate(scales)}
+    names = sorted(graphs)
+    out = {"workload": "R-MAT scales " + "/".join(str(x) for x in scales) +
+                       ", edge factor 16, seeds 5-8; random connected 3-6-vertex patterns; Poisson arrivals",
+           "requests_per_rate": n_req, "rates": {}}
+    with CountServer(graphs, slots=4, max_batch=32) as srv:
+        alone = {nm: srv._dev[nm] for nm in names}
+        for rate in rates:
+            rng = np.random.default_rng(seed + rate)
+            reqs = [Request(i, names[int(rng.integers(0, len(names)))], random_pattern(rng, int(rng.integers(3, 7))))
+                    for i in range(n_req)]
+            offs = poisson_arrivals(rng, float(rate), n_req)
+            for r in reqs:  # plan outside the timed region, as a warm cache would
+                srv.schedule_for(r.graph, r.adj, r.settings)
+            wall = run_load(srv, reqs, offs)
+            lat = [r.latency * 1e3 for r in reqs]
+            # the same requests, each alone, queued serially at the same arrivals
+            service, exact = [], True
+            for r in reqs:
+                t0 = time.perf_counter()
+                one = alone[r.graph].count(srv.schedule_for(r.graph, r.adj, r.settings))
+                service.append(time.perf_counter() - t0)
+                exact = exact and (one.count, one.raw) == (r.count, r.raw)
+            done, slat = 0.0, []
+            for off, sv in zip(offs, service):
+                done = max(done, off) + sv
+                slat.append((done - off) * 1e3)
+            serial_wall = done
+            rec = {"p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+                   "throughput_qps": round(n_req / wall, 2),
+                   "serial_p50_ms": round(percentile(slat, 50), 3), "serial_p99_ms": round(percentile(slat, 99), 3),
+                   "serial_throughput_qps": round(n_req / serial_wall, 2), "exact": exact}
+            log(f"served {rate} q/s: {rec}")
+            assert exact
+            out["rates"][str(rate)] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +178,8 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-balg", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-pattern batched measurement")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--patterns", default=None, help="comma-separated subset (exploration only)")
     ap.add_argument("--scale", type=int, default=None, help="override the R-MAT scale (exploration only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -267,6 +351,9 @@ def main():
             roofline["traffic_frac"] = round(roofline["traffic_GBps"] / HBM_PEAK_GBS, 4)
             roofline["traffic_source"] = t["source"]
 
+    batched = None if args.no_batched else batched_bench(gp)
+    served = None if args.no_served else served_bench(gp)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(host, scheds, cfg["patterns"])
@@ -291,6 +378,8 @@ def main():
         "per_pattern": per_pattern,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

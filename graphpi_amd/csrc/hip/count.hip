@@ -1270,6 +1270,10 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     const int64_t bslot_stride = any_batched && nbslots ? deg_stride : 0;
     const int64_t wave_stride = max_slots * slot_stride + 2 * tmp_stride + (nbslots + 1) * bslot_stride;
     const size_t scratch_bytes = (size_t)std::max<int64_t>(waves * wave_stride, 64) * sizeof(int32_t);
+    // Several plans share one LDS size; the larger pair-level map would cut
+    // residency for all of them, so a batch gives the pair level the batched
+    // level's pool (its sub-batches get smaller instead).
+    if (nplans > 1 && hash_slots > GP_HASH_SLOTS_BATCH) hash_slots = GP_HASH_SLOTS_BATCH;
     // LDS hash pool: tuning->lds_stage elements staged at load factor <= 1/2
     if (tuning && tuning->lds_stage > 0 && hash_slots > 0) {
         hash_slots = 64;
@@ -1540,7 +1544,49 @@ extern "C" int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_
 extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
                               const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats) {
     if (!g || !scheds) return gp_fail(GP_EINVAL, "null argument");
-    return run_plans(const_cast<gp_graph*>(g), scheds, n_scheds, tuning, raw_sums, counts, stats);
+    if (n_scheds < 1 || n_scheds > GP_MAX_BATCH) return gp_fail(GP_ERANGE, "schedule count out of range");
+    if (n_scheds > 1 && tuning && (tuning->task_begin != 0 || tuning->task_end > 0))
+        return gp_fail(GP_EINVAL, "task ranges apply to a single schedule");
+    // One launch per LDS class (pair level / batched level with few or many
+    // leaf blocks / neither): plans of one class share the launch's LDS size
+    // without shrinking the residency of cheaper plans.
+    const bool faithful = tuning && tuning->collect_balg;
+    std::vector<std::vector<int> > groups(6);
+    for (int i = 0; i < n_scheds; ++i) {
+        KPlan kp;
+        int rc = gp_make_kplan(&scheds[i], faithful, !(tuning && tuning->no_batch), !(tuning && tuning->no_pair), &kp);
+        if (rc != GP_OK) return rc;
+        const int cls = kp.pair ? 0 : kp.batched ? (kp.nblocks <= 3 ? 1 : 2) : 3;
+        groups[cls].push_back(i);
+    }
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    for (const std::vector<int>& grp : groups) {
+        if (grp.empty()) continue;
+        std::vector<gp_sched_flat> sub;
+        for (int i : grp) sub.push_back(scheds[i]);
+        std::vector<uint64_t> raw(grp.size());
+        std::vector<int64_t> cnt(grp.size());
+        gp_stats st{};
+        int rc = run_plans(const_cast<gp_graph*>(g), sub.data(), (int)sub.size(), tuning, raw.data(), cnt.data(), &st);
+        if (rc != GP_OK) return rc;
+        for (size_t j = 0; j < grp.size(); ++j) {
+            if (raw_sums) raw_sums[grp[j]] = raw[j];
+            if (counts) counts[grp[j]] = cnt[j];
+        }
+        if (stats) {
+            stats->kernel_ms += st.kernel_ms;
+            stats->balg_bytes += st.balg_bytes;
+            stats->tasks += st.tasks;
+            stats->split_tasks += st.split_tasks;
+            stats->waves = std::max(stats->waves, st.waves);
+            stats->kernel_variant = st.kernel_variant;
+            // finish times of the last group's launch, offset by the groups before it
+            stats->finish_p50_ms = stats->kernel_ms - st.kernel_ms + st.finish_p50_ms;
+            stats->finish_p90_ms = stats->kernel_ms - st.kernel_ms + st.finish_p90_ms;
+            stats->finish_max_ms = stats->kernel_ms - st.kernel_ms + st.finish_max_ms;
+        }
+    }
+    return GP_OK;
 }
 
 extern "C" int gp_count_footprint(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,

@@ -141,10 +141,12 @@ GP_API int gp_graph_bytes(const gp_graph* g, uint64_t* bytes);
 GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
                     uint64_t* raw_sum, int64_t* count, gp_stats* stats);
 
-/* Count n_scheds schedules over one uploaded graph in one persistent launch
- * (plus one launch for the parts of deferred heavy tasks): the work items of
- * every schedule share one work counter, so schedules of uneven cost balance
- * across the device.  raw_sums[i] / counts[i] equal what gp_count returns
+/* Count n_scheds schedules over one uploaded graph with persistent launches
+ * shared by the schedules: the schedules are grouped by the LDS their kernel
+ * plan needs (at most four groups), and within a group the work items of
+ * every schedule share one work counter in one launch (plus one launch for the
+ * parts of deferred heavy tasks), so schedules of uneven cost balance across
+ * the device.  raw_sums[i] / counts[i] equal what gp_count returns
  * for scheds[i]; stats (optional) covers the whole batch.  tuning applies to
  * every schedule; a task range is rejected (GP_EINVAL) when n_scheds > 1.
  * At most 256 schedules per call.  No reference counterpart: it serves the
