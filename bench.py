@@ -35,6 +35,13 @@ CONFIGS = {
         "patterns": ["4clique"],
         "named": ["4clique", "pentagon", "house"],
     },
+    "cfg3": {
+        "workload": "Chung-Lu 3,000,000 vertices, Pareto exponent 2.1, 117M sampled edges, max expected degree "
+                    "30000, seed 3; pm=1 rt=1 IEP on",
+        "gen": ("chunglu", 3000000, 117000000, 3),
+        "patterns": ["4clique"],
+        "named": ["4clique", "5clique", "tri_edge_tri"],
+    },
     "cfg1": {
         "workload": "R-MAT scale 13, edge factor 12, (0.57,0.19,0.19,0.05), seed 1; pm=1 rt=1 IEP on",
         "gen": ("rmat", 13, 12, 1),
@@ -209,7 +216,10 @@ def main():
         scale = args.scale
         cfg["workload"] = cfg["workload"].replace("scale %d" % cfg["gen"][1], "scale %d" % scale)
     t0 = time.perf_counter()
-    host = gp.HostGraph.rmat(scale, ef, 0.57, 0.19, 0.19, seed)
+    if kind == "chunglu":
+        host = gp.HostGraph.chung_lu(scale, ef, 2.1, 30000.0, seed)
+    else:
+        host = gp.HostGraph.rmat(scale, ef, 0.57, 0.19, 0.19, seed)
     log(f"graph: {host.v_cnt} vertices, {host.e_cnt // 2} undirected edges, max degree {host.max_degree} "
         f"({time.perf_counter() - t0:.1f} s)")
     dev = gp.DeviceGraph(host)
@@ -223,18 +233,19 @@ def main():
 
     totals = {name: dev.count_tasks(scheds[name]) for name in cfg["patterns"]}
 
-    def count_pattern(name, target_ms=20000.0, tuning=None, quiet=False):
-        """One full count of `name` as consecutive task ranges: a first range
-        of 1/64 of the tasks; then the rest in one launch when that is
-        predicted to take under a minute, else in ranges sized for about
-        target_ms each (progress is logged per range).  The raw sums of the
-        ranges add up to the whole count."""
+    def count_pattern(name, target_ms=15000.0, tuning=None, quiet=False):
+        """One full count of `name` as consecutive task ranges (progress is
+        logged per range): a first range of 1/16 of the tasks, then ranges at
+        most 4 times the previous one and sized for about target_ms at the
+        previous range's rate (task cost varies along the task order, so each
+        launch stays bounded).  The raw sums of the ranges add up to the whole
+        count."""
         s = scheds[name]
         total = totals[name]
         tun = gp.Tuning()
         if tuning is not None:
             ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
-        first = total if total <= (1 << 20) else max(1, total // 64)
+        first = total if total <= (1 << 20) else max(1, total // 16)
         begin, size, raw, ms, launches = 0, first, 0, 0.0, 0
         balg_sum = 0
         t0 = time.perf_counter()
@@ -251,11 +262,10 @@ def main():
             if not quiet:
                 log(f"    {name}: tasks {begin}..{done} of {total}: {k:.1f} ms (tail: half {r.stats.finish_p50_ms:.1f}, "
                     f"all {r.stats.finish_max_ms:.1f} ms; {r.stats.split_tasks} tasks split)")
-            rate = max(k, 0.01) / max(1, done - begin)   # ms per task so far
-            if launches == 1 and rate * (total - done) < 60000.0:
+            rate = max(k, 0.01) / max(1, done - begin)   # ms per task in the last range
+            size = max(1, min(4 * (done - begin), int(target_ms / rate)))
+            if total - done <= size + size // 4:
                 size = total - done
-            else:
-                size = max(1, int(target_ms / rate))
             begin = done
         signed = raw - (1 << 64) if raw >= (1 << 63) else raw
         red = s.redundancy or 1
