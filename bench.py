@@ -187,6 +187,8 @@ def main():
     ap.add_argument("--no-balg", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-pattern batched measurement")
     ap.add_argument("--no-served", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick launch settings per pattern on a task sample before the timed region")
     ap.add_argument("--patterns", default=None, help="comma-separated subset (exploration only)")
     ap.add_argument("--scale", type=int, default=None, help="override the R-MAT scale (exploration only)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -232,6 +234,11 @@ def main():
         scheds[name] = s
 
     totals = {name: dev.count_tasks(scheds[name]) for name in cfg["patterns"]}
+    tunings, tune_reports = {}, {}
+    if args.autotune:
+        from graphpi_amd.autotune import autotune
+        for name in cfg["patterns"]:
+            tunings[name], tune_reports[name] = autotune(dev, scheds[name], log=log)
 
     def count_pattern(name, target_ms=15000.0, tuning=None, quiet=False):
         """One full count of `name` as consecutive task ranges (progress is
@@ -276,7 +283,7 @@ def main():
         out = {}
         for name in cfg["patterns"]:
             log(f"  counting {name} ...")
-            out[name] = count_pattern(name)
+            out[name] = count_pattern(name, tuning=tunings.get(name))
             log(f"  {name}: {out[name]['count']} in {out[name]['kernel_ms']:.1f} ms kernel time "
                 f"({out[name]['launches']} launches)")
         return out
@@ -342,6 +349,8 @@ def main():
         traffic = {}
     for n in cfg["patterns"]:
         per_pattern[n] = {"count": counts[n], "kernel_ms": round(kernel_ms[n], 3), "launches": launches[n]}
+        if n in tune_reports:
+            per_pattern[n]["autotune"] = tune_reports[n]
         if n in balg:
             per_pattern[n]["balg_bytes"] = balg[n]
             per_pattern[n]["balg_GBps"] = round(balg[n] / (kernel_ms[n] * 1e-3) / 1e9, 1)

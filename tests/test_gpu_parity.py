@@ -388,3 +388,15 @@ def test_batch_rejects_bad_input(rmat13_path):
         dev.count_batch([])
     with pytest.raises(gp.GPError):
         dev.count_batch([s] * 257)
+
+
+def test_autotune_keeps_counts(rmat13_path):
+    from graphpi_amd.autotune import autotune
+    host = gp.HostGraph.load_text(rmat13_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    for name in ["house", "4clique"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        tun, rep = autotune(dev, s)
+        assert rep["sample_ms_tuned"] <= rep["sample_ms_default"]
+        assert dev.count(s, tun).raw == dev.count(s).raw, (name, rep)
