@@ -98,6 +98,8 @@ class Tuning(ctypes.Structure):
         ("split_min", ctypes.c_int32),
         ("task_begin", ctypes.c_int64),
         ("task_end", ctypes.c_int64),
+        ("pair_map", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -56,6 +56,7 @@
 #define GP_SPLIT_PARTS 64
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
+#define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
 
 namespace {
 
@@ -77,6 +78,8 @@ struct KArgs {
     int64_t wave_stride;
     int64_t tmp_stride;
     int64_t bslot_stride;
+    int32_t* dense;          // pair level: per-wave count array over all vertex ids (kept zero), or null
+    int64_t dense_stride;
     int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
     int32_t hash_lg;
     int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
@@ -738,7 +741,128 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     // sub-batch and each element adds its count.  Pairs with v == a, which the
     // loop nest skips, are subtracted.  An a whose set is too large for the
     // map goes through the batched leaf level on its own.
+    // Pair level with a dense per-wave count array cnt[] over all vertex ids
+    // instead of the LDS map: scatter every kept element of every N(a) once
+    // (cnt[x] += 1), stream every candidate list N(v) once summing cnt[x],
+    // subtract the pairs v == a, then clear what was scattered.  Each list is
+    // read once or twice, where the LDS map re-streams the N(v) lists once per
+    // sub-batch of a's.  Same filters and result as pair_level below.
+    auto pair_level_dense = [&]() {
+        const int da = L - 2, dv = L - 1;
+        const int32_t astart = da == 2 ? dstart : 0, astep = da == 2 ? dstride : 1;
+        const int32_t* D = s_lp[da] + astart;
+        const int32_t Dn = s_ll[da] > astart ? (s_ll[da] - astart + astep - 1) / astep : 0;
+        if (Dn <= 0) return;
+        const int qc = pl->loop_set[dv];
+        const int32_t* C = s_ptr[qc];
+        int32_t Cn = s_len[qc];
+        if (pl->rmask[dv]) Cn = wave_lower_bound(C, Cn, bound(pl->rmask[dv]));
+        if (Cn <= 0) return;
+        const int32_t b = bound(pl->rmask[L]);
+        const int gpf = pl->pair_g;
+        const int32_t* gptr = gpf >= 0 ? s_ptr[gpf] : nullptr;
+        const int32_t glen = gpf >= 0 ? s_len[gpf] : 0;
+        int32_t* const cnt = a.dense + (int64_t)blockIdx.x * a.dense_stride;
+        const gcptr E = GL(a.edge);
+        // mode 0: scatter kept elements; 1: gather counts; 2: clear
+        auto visit = [&](int mode, int32_t x, bool act) {
+            if (mode == 0) {
+                bool keep = act && x < b;
+                for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
+                if (keep && gpf >= 0) keep = contains(gptr, glen, x);
+                if (keep) atomicAdd(&cnt[x], 1);
+            } else if (mode == 1) {
+                if (act && x < b) lacc += (uint64_t)__hip_atomic_load(&cnt[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                if (act && x < b) cnt[x] = 0;
+            }
+        };
+        // every list N(u) ∩ [0, b) of the 64 lanes' vertices u (u < 0: none)
+        auto stream = [&](int mode, int32_t u) {
+            int32_t us = 0, ucut = 0;
+            if (u >= 0) {
+                const uint32_t l = a.vertex[u];
+                us = (int32_t)l;
+                const int32_t dg = (int32_t)(a.vertex[u + 1] - l);
+                ucut = (b >= v_cnt || dg <= 16) ? dg : lower_bound_lane(a.edge + l, dg, b);
+            }
+            const bool lng = ucut >= GP_LONG_LIST;
+            int32_t tot = 0;
+            const int32_t ex = wave_excl_scan(lng ? 0 : ucut, &tot);
+            s_off2[lane] = ex;
+            __syncthreads();
+            for (int32_t f0 = 0; f0 < tot; f0 += WAVE) {
+                const int32_t f = f0 + lane;
+                const bool act = f < tot;
+                int lo = 0, hi = WAVE;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_off2[mid] <= f) lo = mid; else hi = mid;
+                }
+                const int32_t os = __shfl(us, lo);
+                const int32_t oe = __shfl(ex, lo);
+                visit(mode, act ? E[os + (f - oe)] : -1, act);
+            }
+            uint64_t lm = __ballot(lng);
+            while (lm) {
+                const int j = __ffsll((long long)lm) - 1;
+                lm &= lm - 1;
+                const gcptr pv = E + __shfl(us, j);
+                const int32_t pn = __shfl(ucut, j);
+                for (int32_t f0 = 0; f0 < pn; f0 += GP_UNROLL * WAVE) {
+                    int32_t x[GP_UNROLL];
+#pragma unroll
+                    for (int q = 0; q < GP_UNROLL; ++q) x[q] = f0 + q * WAVE + lane < pn ? pv[f0 + q * WAVE + lane] : -1;
+#pragma unroll
+                    for (int q = 0; q < GP_UNROLL; ++q) visit(mode, x[q], x[q] >= 0);
+                }
+            }
+            __syncthreads();
+        };
+        auto a_of = [&](int32_t ai) -> int32_t {
+            bool va = ai < Dn;
+            const int32_t av = va ? D[(int64_t)ai * astep] : -1;
+            for (int j = 0; j < da; ++j) va = va && av != s_emb[j];
+            return va ? av : -1;
+        };
+        for (int32_t abase = 0; abase < Dn; abase += WAVE) stream(0, a_of(abase + lane));
+        __threadfence();
+        for (int32_t cbase = 0; cbase < Cn; cbase += WAVE) {
+            const int32_t ci = cbase + lane;
+            bool vc = ci < Cn;
+            const int32_t vv = vc ? C[ci] : -1;
+            for (int j = 0; j < da; ++j) vc = vc && vv != s_emb[j];
+            stream(1, vc ? vv : -1);
+        }
+        // pairs v == a (the loop nest skips them): each kept element of N(a) once
+        for (int32_t abase = 0; abase < Dn; abase += WAVE) {
+            const int32_t av = a_of(abase + lane);
+            if (av >= 0 && contains(C, Cn, av)) {
+                const uint32_t l = a.vertex[av];
+                const int32_t dg = (int32_t)(a.vertex[av + 1] - l);
+                const gcptr na = E + l;
+                uint64_t own = 0;
+                for (int32_t i = 0; i < dg; ++i) {
+                    const int32_t x = na[i];
+                    if (x >= b) break;
+                    bool keep = true;
+                    for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
+                    if (keep && gpf >= 0) keep = contains(gptr, glen, x);
+                    own += keep ? 1 : 0;
+                }
+                lacc -= own;
+            }
+        }
+        __threadfence();
+        for (int32_t abase = 0; abase < Dn; abase += WAVE) stream(2, a_of(abase + lane));
+        __threadfence();
+    };
+
     auto pair_level = [&]() {
+        if (a.dense) {
+            pair_level_dense();
+            return;
+        }
         const int da = L - 2, dv = L - 1;
         const int32_t astart = da == 2 ? dstart : 0, astep = da == 2 ? dstride : 1;
         const int32_t* D = s_lp[da] + astart;
@@ -1068,6 +1192,11 @@ __global__ void build_src_kernel(const uint32_t* __restrict__ vertex, int32_t v_
     src[e] = lo;
 }
 
+__global__ void zero_i32_kernel(int32_t* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 __global__ void fill_u64_kernel(unsigned long long* p, int n) {
     for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0;
 }
@@ -1090,6 +1219,8 @@ struct CallCtx {
     unsigned long long* d_heavy = nullptr; // GP_HEAVY_CAP deferred tasks
     int32_t* d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    int32_t* d_dense = nullptr; // pair level counts, all zero between calls
+    size_t dense_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -1123,6 +1254,7 @@ void destroy_ctx(CallCtx* c) {
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_heavy) (void)hipFree(c->d_heavy);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
+    if (c->d_dense) (void)hipFree(c->d_dense);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1194,11 +1326,14 @@ struct Launch {
     size_t scratch_bytes = 0;
     int hash_slots = 0, hash_lg = 0;
     size_t lds_bytes = 0;
-    // device memory the call needs besides the graph: scratch, plans and
-    // work layout, counters, clock stamps and the deferred-task list
+    int64_t dense_stride = 0; // ints per wave of the pair level's count array (0: LDS map)
+    size_t dense_bytes = 0;
+    // device memory the call needs besides the graph: scratch, dense pair
+    // counts, plans and work layout, counters, clock stamps and the
+    // deferred-task list
     size_t device_bytes() const {
         const size_t n = kps.size();
-        return scratch_bytes + sizeof(KPlan) * n + sizeof(uint64_t) * (3 * n + 1) +
+        return scratch_bytes + dense_bytes + sizeof(KPlan) * n + sizeof(uint64_t) * (3 * n + 1) +
                sizeof(unsigned long long) * (3 + 3 * n + 4 * (size_t)waves) + sizeof(unsigned long long) * GP_HEAVY_CAP;
     }
 };
@@ -1282,6 +1417,19 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     int hash_lg = 0;
     while ((1 << hash_lg) < hash_slots) ++hash_lg;
     const size_t lds_bytes = sizeof(int32_t) * ((size_t)hash_slots + (size_t)bc_rows * WAVE);
+    // pair level: a dense count array per wave when every wave's fits the cap
+    bool any_pair = false;
+    for (const KPlan& kp : kps) any_pair = any_pair || kp.pair;
+    int64_t dense_stride = 0;
+    size_t dense_bytes = 0;
+    if (any_pair && !(tuning && tuning->pair_map == 1)) {
+        const int64_t stride = ((int64_t)std::max(g->v_cnt, 1) + 63) / 64 * 64;
+        const size_t bytes = (size_t)waves * (size_t)stride * sizeof(int32_t);
+        if (bytes <= GP_DENSE_MAX_BYTES) {
+            dense_stride = stride;
+            dense_bytes = bytes;
+        }
+    }
 
     out->faithful = faithful;
     out->any_split = any_split;
@@ -1300,6 +1448,8 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     out->hash_slots = hash_slots;
     out->hash_lg = hash_lg;
     out->lds_bytes = lds_bytes;
+    out->dense_stride = dense_stride;
+    out->dense_bytes = dense_bytes;
     return GP_OK;
 }
 
@@ -1325,6 +1475,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     const size_t scratch_bytes = ln.scratch_bytes;
     const int hash_slots = ln.hash_slots, hash_lg = ln.hash_lg;
     const size_t lds_bytes = ln.lds_bytes;
+    const int64_t dense_stride = ln.dense_stride;
+    const size_t dense_bytes = ln.dense_bytes;
     CallCtx* c = nullptr;
     int rc = acquire_ctx(g, scratch_bytes, &c);
     if (rc != GP_OK) return rc;
@@ -1333,6 +1485,15 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     const size_t n_cnt = 3 + 3 * (size_t)nplans;
     hipError_t e = ensure((void**)&c->d_meta, &c->meta_cap, meta_bytes);
     if (e == hipSuccess) e = ensure((void**)&c->d_cnt, &c->cnt_cap, n_cnt * sizeof(unsigned long long));
+    if (e == hipSuccess && dense_bytes > c->dense_bytes) {
+        // a new array starts at zero; the kernels leave it at zero
+        e = ensure((void**)&c->d_dense, &c->dense_bytes, dense_bytes);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(zero_i32_kernel, dim3(4096), dim3(256), 0, c->stream, c->d_dense,
+                               (uint64_t)(c->dense_bytes / sizeof(int32_t)));
+            e = hipGetLastError();
+        }
+    }
     if (e == hipSuccess && c->clocks_cap < (size_t)waves) {
         // start and finish stamps per wave, for each of the two launches
         size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
@@ -1374,6 +1535,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.bslot_stride = bslot_stride;
     a.hash_slots = hash_slots;
     a.hash_lg = hash_lg;
+    a.dense = dense_bytes ? c->d_dense : nullptr;
+    a.dense_stride = dense_stride;
     a.clocks = c->d_clocks;
     std::vector<unsigned long long> host_out(n_cnt, 0);
     std::vector<unsigned long long> clocks(4 * (size_t)waves, 0);

@@ -105,6 +105,9 @@ typedef struct gp_tuning {
      * disjoint ranges add up to the raw sum of the whole count. */
     int64_t task_begin;
     int64_t task_end;
+    int32_t pair_map;       /* last two levels as a pair: 0 = dense per-wave count array when
+                               it fits (64 GiB over all waves), else the LDS map; 1 = LDS map */
+    int32_t reserved;
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */

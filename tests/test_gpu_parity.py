@@ -145,14 +145,16 @@ def test_pair_level_matches(name):
     no_pair.no_pair = 1
     no_batch = gp.Tuning()
     no_batch.no_batch = 1
+    lds_map = gp.Tuning()  # the pair level's LDS count map instead of the dense array
+    lds_map.pair_map = 1
     for pm, rt, iep in [(1, 1, 1), (1, 1, 0), (0, 1, 0)]:
         s, valid = gp.build_schedule(gp.PATTERNS[name], pm, rt, iep, host.v_cnt, host.e_cnt, tri)
         if not valid:
             continue
-        a, b, c = dev.count(s), dev.count(s, no_pair), dev.count(s, no_batch)
-        print(f"{name} ({pm},{rt},{iep}) pair {a.stats.kernel_ms:.2f} ms batched {b.stats.kernel_ms:.2f} ms "
-              f"per-candidate {c.stats.kernel_ms:.2f} ms", flush=True)
-        assert a.raw == b.raw == c.raw, (name, pm, rt, iep)
+        a, b, c, m = dev.count(s), dev.count(s, no_pair), dev.count(s, no_batch), dev.count(s, lds_map)
+        print(f"{name} ({pm},{rt},{iep}) pair dense {a.stats.kernel_ms:.2f} ms, LDS map {m.stats.kernel_ms:.2f} ms, "
+              f"batched {b.stats.kernel_ms:.2f} ms, per-candidate {c.stats.kernel_ms:.2f} ms", flush=True)
+        assert a.raw == b.raw == c.raw == m.raw, (name, pm, rt, iep)
     small = _random_graph(150, 0.06, 9, hubs=4)
     ds = gp.DeviceGraph(small)
     s = _sched(small, gp.PATTERNS[name], 1, 1, 1)
