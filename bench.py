@@ -240,13 +240,14 @@ def main():
         for name in cfg["patterns"]:
             tunings[name], tune_reports[name] = autotune(dev, scheds[name], log=log)
 
-    def count_pattern(name, target_ms=15000.0, tuning=None, quiet=False):
+    def count_pattern(name, target_ms=5000.0, tuning=None, quiet=False):
         """One full count of `name` as consecutive task ranges (progress is
-        logged per range): a first range of 1/16 of the tasks, then ranges at
-        most 4 times the previous one and sized for about target_ms at the
-        previous range's rate (task cost varies along the task order, so each
-        launch stays bounded).  The raw sums of the ranges add up to the whole
-        count."""
+        logged per range): a first range of 1/16 of the tasks; the rest in one
+        launch when that is predicted to take under 10 s even at 4 times the
+        first range's rate; otherwise ranges at most twice the previous one and
+        sized for about target_ms at the previous range's rate (task cost
+        varies along the task order, so each launch stays bounded).  The raw
+        sums of the ranges add up to the whole count."""
         s = scheds[name]
         total = totals[name]
         tun = gp.Tuning()
@@ -270,8 +271,8 @@ def main():
                 log(f"    {name}: tasks {begin}..{done} of {total}: {k:.1f} ms (tail: half {r.stats.finish_p50_ms:.1f}, "
                     f"all {r.stats.finish_max_ms:.1f} ms; {r.stats.split_tasks} tasks split)")
             rate = max(k, 0.01) / max(1, done - begin)   # ms per task in the last range
-            size = max(1, min(4 * (done - begin), int(target_ms / rate)))
-            if total - done <= size + size // 4:
+            size = max(1, min(2 * (done - begin), int(target_ms / rate)))
+            if total - done <= size + size // 4 or (launches == 1 and 4 * rate * (total - done) < 10000.0):
                 size = total - done
             begin = done
         signed = raw - (1 << 64) if raw >= (1 << 63) else raw
