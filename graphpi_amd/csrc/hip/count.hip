@@ -320,6 +320,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kinds 0 and 1)
     __shared__ int32_t s_off[WAVE + 1];
     __shared__ int32_t s_off2[WAVE + 1];
+    __shared__ int32_t s_plane[WAVE]; // lane of the r-th probed candidate
 
     // the plan of the current work item; switching plans flushes the sums
     int cur_plan = 0;
@@ -585,9 +586,11 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             // streamed part of N(v): elements below b and below max(G)+1
             const int32_t scut = smax < 0 ? -1 : min(b, smax + 1);
             int32_t work = 0;
+            int32_t pcut = 0; // probed lists: elements below scut
             bool probe = false;
             if (valid && nstream > 0 && scut >= 0) {
                 int32_t cut = deg <= 16 ? deg : lower_bound_lane(nv, deg, scut);
+                pcut = cut;
                 // stream the list, or probe the sets into it when cheaper
                 const int64_t stream_cost = (int64_t)cut * stream_cost_w;
                 const int64_t probe_cost = (int64_t)stream_total * (ilog2_ceil(cut + 1) + 1) * 2;
@@ -683,18 +686,38 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     if (lane == j) s_bc[b2][j] = c;
                 }
             }
-            // candidates whose lists are probed
-            uint64_t pm = __ballot(probe);
-            while (pm) {
-                const int j = __ffsll((long long)pm) - 1;
-                pm &= pm - 1;
-                const int32_t* pv = a.edge + __shfl(nvs, j);
-                const int32_t pd = __shfl(deg, j);
-                const int32_t pc = __shfl(scut, j);
+            // candidates whose lists are probed: every (candidate, element of
+            // G) pair is one binary search, and the pairs of all probed
+            // candidates are spread over the lanes together
+            const uint64_t pm = __ballot(probe);
+            if (pm) {
+                const int np = popc64(pm);
+                if (probe) s_plane[popc64(pm & ((1ull << lane) - 1))] = lane;
+                __syncthreads();
                 for (int b2 = 0; b2 < nb; ++b2) {
                     if (pl->bl_kind[b2] != 1) continue;
-                    const int32_t c = intersect<true>(s_bg[b2], s_bn[b2], pv, pd, pc, nullptr, s_emb, d);
-                    if (lane == j) s_bc[b2][j] = c;
+                    const int32_t gn = s_bn[b2];
+                    const gcptr g = GL(s_bg[b2]);
+                    const int h = s_bh[b2];
+                    if (probe) s_bc[b2][lane] = 0;
+                    __syncthreads();
+                    const int32_t items = np * gn;
+                    for (int32_t t0 = 0; t0 < items; t0 += WAVE) {
+                        const int32_t t = t0 + lane;
+                        const bool act = t < items;
+                        const int32_t r = act ? t / gn : 0;
+                        const int cl = act ? s_plane[r] : lane;
+                        const int32_t o_nvs = __shfl(nvs, cl);
+                        const int32_t o_cut = __shfl(pcut, cl);
+                        bool hit = false;
+                        if (act) {
+                            const int32_t i = t - r * gn;
+                            const int32_t x = h >= 0 ? s_hash[h + i] : g[i];
+                            hit = contains(a.edge + o_nvs, o_cut, x);
+                            for (int j = 0; j < d; ++j) hit = hit && x != s_emb[j];
+                        }
+                        if (hit) atomicAdd(&s_bc[b2][cl], 1);
+                    }
                 }
             }
             __syncthreads();
