@@ -99,7 +99,7 @@ class Tuning(ctypes.Structure):
         ("task_begin", ctypes.c_int64),
         ("task_end", ctypes.c_int64),
         ("pair_map", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("probe_pct", ctypes.c_int32),
     ]
 
 

@@ -23,6 +23,7 @@ KNOBS = {
     "chunk": [0, 1, 4, 16, 64],
     "split_min": [0, 128, 512, 4096, -1],
     "lds_stage": [0, 128, 256],
+    "probe_pct": [0, 25, 50, 200, 400],
 }
 
 

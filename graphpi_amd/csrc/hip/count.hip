@@ -80,6 +80,7 @@ struct KArgs {
     int64_t bslot_stride;
     int32_t* dense;          // pair level: per-wave count array over all vertex ids (kept zero), or null
     int64_t dense_stride;
+    int32_t probe_pct;    // probing is chosen when its cost x probe_pct / 100 is below streaming's
     int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
     int32_t hash_lg;
     int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
@@ -594,7 +595,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 // stream the list, or probe the sets into it when cheaper
                 const int64_t stream_cost = (int64_t)cut * stream_cost_w;
                 const int64_t probe_cost = (int64_t)stream_total * (ilog2_ceil(cut + 1) + 1) * 2;
-                probe = cut > 64 && probe_cost < stream_cost;
+                probe = cut > 64 && probe_cost * a.probe_pct < stream_cost * 100;
                 work = probe ? 0 : cut;
             }
             // long lists go through the whole-wave path below
@@ -1558,6 +1559,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.bslot_stride = bslot_stride;
     a.hash_slots = hash_slots;
     a.hash_lg = hash_lg;
+    a.probe_pct = tuning && tuning->probe_pct > 0 ? tuning->probe_pct : 100;
     a.dense = dense_bytes ? c->d_dense : nullptr;
     a.dense_stride = dense_stride;
     a.clocks = c->d_clocks;

@@ -107,7 +107,9 @@ typedef struct gp_tuning {
     int64_t task_end;
     int32_t pair_map;       /* last two levels as a pair: 0 = dense per-wave count array when
                                it fits (64 GiB over all waves), else the LDS map; 1 = LDS map */
-    int32_t reserved;
+    int32_t probe_pct;      /* batched level: a long candidate list is probed instead of streamed
+                               when probing's modelled cost x probe_pct / 100 is lower;
+                               0 = default (100) */
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */
