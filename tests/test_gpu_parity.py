@@ -402,3 +402,20 @@ def test_autotune_keeps_counts(rmat13_path):
         tun, rep = autotune(dev, s)
         assert rep["sample_ms_tuned"] <= rep["sample_ms_default"]
         assert dev.count(s, tun).raw == dev.count(s).raw, (name, rep)
+
+
+def test_chung_lu_skewed_graph_matches_oracle():
+    # power-law degrees with a few hubs: the skew the deferred-task and
+    # probing paths exist for
+    host = gp.HostGraph.chung_lu(4000, 40000, 2.1, 600.0, 3)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    assert dev.triangle_counting() == tri
+    for name in ["4clique", "5clique", "house", "pentagon", "rectangle", "qg3"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        want = ob.pattern_matching(host, s, threads=8)
+        for split in (0, 1):
+            tun = gp.Tuning()
+            tun.split_min = split
+            got = dev.count(s, tun)
+            assert (got.raw, got.count) == (want["raw"], want["count"]), (name, split)
