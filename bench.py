@@ -77,15 +77,18 @@ def cpu_baseline(host, scheds, names, budget_s=20.0):
     v = host.v_cnt
     for name in names:
         s = scheds[name]
-        stride = max(1, v // 64)
+        # start from 1/4096 of the roots and widen towards the time budget
+        stride = max(1, v // 4096)
         t_sample = 0.0
-        for _ in range(4):
+        for _ in range(6):
             t0 = time.perf_counter()
             ob.pattern_matching(host, s, threads=threads, root_begin=stride // 2, root_end=-1, root_stride=stride)
             t_sample = time.perf_counter() - t0
+            log(f"cpu sample {name}: stride {stride}, {t_sample:.2f} s")
             if t_sample >= per / 4 or stride == 1:
                 break
-            stride = max(1, int(stride * max(t_sample, 1e-3) / per))
+            # at most 16 times the roots per step: cost is uneven over the roots
+            stride = max(1, stride // 16, min(stride // 2, int(stride * max(t_sample, 1e-3) / per)))
         est = t_sample * stride
         total_est += est
         samples.append(f"{name}: roots v = {stride // 2} + {stride}*i ({(v + stride - 1) // stride} roots), "
@@ -123,11 +126,23 @@ def batched_bench(gp, scale=10):
     exact = all(r.raw == x for r, x in zip(res, singles))
     log(f"batched: one launch {batch_wall:.2f} s ({st.kernel_ms:.0f} ms kernel), identical={exact}")
     assert exact
+    # the same batch as hipGraph launches: built on the first call, replayed
+    # (kernel arguments updated) on the second
+    gt = gp.Tuning()
+    gt.use_graph = 1
+    dev.count_batch(scheds, gt)
+    t0 = time.perf_counter()
+    gres, gst = dev.count_batch(scheds, gt)
+    graph_wall = time.perf_counter() - t0
+    gexact = all(r.raw == x for r, x in zip(gres, singles))
+    log(f"batched: graph replay {graph_wall:.2f} s ({gst.kernel_ms:.0f} ms), identical={gexact}")
+    assert gexact
     n = len(scheds)
     return {"workload": f"R-MAT scale {scale}, edge factor 16, seed 4; all {n} connected 5- and 6-vertex patterns",
             "v_cnt": host.v_cnt, "undirected_edges": host.e_cnt // 2,
             "patterns_per_s_batched": round(n / batch_wall, 3), "patterns_per_s_one_at_a_time": round(n / loop_wall, 3),
             "batched_s": round(batch_wall, 4), "one_at_a_time_s": round(loop_wall, 4),
+            "batched_graph_replay_s": round(graph_wall, 4), "patterns_per_s_graph_replay": round(n / graph_wall, 3),
             "speedup": round(loop_wall / batch_wall, 3), "exact": exact}
 
 
@@ -189,8 +204,15 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--autotune", action="store_true",
                     help="pick launch settings per pattern on a task sample before the timed region")
+    ap.add_argument("--batched-scale", type=int, default=10,
+                    help="R-MAT scale of the many-pattern batched measurement (default 10)")
+    ap.add_argument("--served-scales", default="8,9,10,11",
+                    help="R-MAT scales of the served graphs (default 8,9,10,11)")
     ap.add_argument("--patterns", default=None, help="comma-separated subset (exploration only)")
     ap.add_argument("--scale", type=int, default=None, help="override the R-MAT scale (exploration only)")
+    ap.add_argument("--sample-frac", type=float, default=0.0, metavar="F",
+                    help="exploration only: count 64 task ranges spread over the task space, covering a fraction F "
+                         "of the tasks, and print the full count's estimated kernel time (no count is reported)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -234,6 +256,26 @@ def main():
         scheds[name] = s
 
     totals = {name: dev.count_tasks(scheds[name]) for name in cfg["patterns"]}
+    if args.sample_frac > 0:
+        est = {}
+        for name in cfg["patterns"]:
+            total, parts = totals[name], 64
+            width = max(1, int(total * args.sample_frac / parts))
+            tun, ms, sampled = gp.Tuning(), 0.0, 0
+            for i in range(parts):
+                tun.task_begin = total * i // parts
+                tun.task_end = min(total, tun.task_begin + width)
+                r = dev.count(scheds[name], tun)
+                ms += r.stats.kernel_ms
+                sampled += tun.task_end - tun.task_begin
+                log(f"  {name} sample {i + 1}/{parts}: tasks {tun.task_begin}..{tun.task_end}: "
+                    f"{r.stats.kernel_ms:.1f} ms")
+            est[name] = {"sampled_tasks": sampled, "total_tasks": total, "sample_kernel_ms": round(ms, 3),
+                         "estimated_full_s": round(ms * total / sampled / 1e3, 1)}
+            log(f"  {name}: estimated {est[name]['estimated_full_s']} s for the full count")
+        print(json.dumps({"mode": "estimate (sampled task ranges; not a count)", "workload": cfg["workload"],
+                          "v_cnt": host.v_cnt, "undirected_edges": host.e_cnt // 2, "estimates": est}), flush=True)
+        return
     tunings, tune_reports = {}, {}
     if args.autotune:
         from graphpi_amd.autotune import autotune
@@ -242,7 +284,7 @@ def main():
 
     def count_pattern(name, target_ms=5000.0, tuning=None, quiet=False):
         """One full count of `name` as consecutive task ranges (progress is
-        logged per range): a first range of 1/16 of the tasks; the rest in one
+        logged per range): a first range of 1/256 of the tasks; the rest in one
         launch when that is predicted to take under 10 s even at 4 times the
         first range's rate; otherwise ranges at most twice the previous one and
         sized for about target_ms at the previous range's rate (task cost
@@ -253,7 +295,7 @@ def main():
         tun = gp.Tuning()
         if tuning is not None:
             ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
-        first = total if total <= (1 << 20) else max(1, total // 16)
+        first = total if total <= (1 << 20) else max(1, total // 256)
         begin, size, raw, ms, launches = 0, first, 0, 0.0, 0
         balg_sum = 0
         t0 = time.perf_counter()
@@ -371,8 +413,9 @@ def main():
             roofline["traffic_frac"] = round(roofline["traffic_GBps"] / HBM_PEAK_GBS, 4)
             roofline["traffic_source"] = t["source"]
 
-    batched = None if args.no_batched else batched_bench(gp)
-    served = None if args.no_served else served_bench(gp)
+    batched = None if args.no_batched else batched_bench(gp, scale=args.batched_scale)
+    served = None if args.no_served else served_bench(
+        gp, scales=tuple(int(x) for x in args.served_scales.split(",")))
 
     cpu = None
     if not args.no_cpu:

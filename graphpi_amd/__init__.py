@@ -100,6 +100,9 @@ class Tuning(ctypes.Structure):
         ("task_end", ctypes.c_int64),
         ("pair_map", ctypes.c_int32),
         ("probe_pct", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("no_shared_search", ctypes.c_int32),
+        ("no_pair_v1", ctypes.c_int32),
     ]
 
 

@@ -57,6 +57,8 @@
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 #define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
+#define GP_FLAG_VMAX ((1 << 23) - 1) // flagged LDS tables pack vertex ids below this with 8 flag bits
+#define GP_GRAPH_CACHE 8          // instantiated launch graphs kept per call context
 
 namespace {
 
@@ -215,6 +217,11 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
     return (uint64_t)hi << 32 | lo;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) {
+    for (int off = WAVE / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    return x;
+}
+
 // lane-private lower bound (first index with a[i] >= x)
 __device__ __forceinline__ int32_t lower_bound_lane(const int32_t* __restrict__ ap, int32_t n, int32_t x) {
     const gcptr a = GL(ap);
@@ -297,10 +304,44 @@ __device__ __forceinline__ uint32_t staged_has_n(const int32_t* tab, int lg, con
     return hits;
 }
 
+// Flagged tables: entry = value << 8 | flags, sorted by value, padded with
+// INT_MAX (value 2^23 - 1, never a vertex id below GP_FLAG_VMAX).  A lookup
+// searches x << 8; the result is the flags of x, or 0 when x is absent.
+__device__ __forceinline__ uint32_t staged_flags(const int32_t* tab, int lg, int32_t x) {
+    const int32_t key = x < 0 ? INT_MIN : (int32_t)((uint32_t)x << 8);
+    int32_t pos = 0;
+    for (int32_t step = (1 << lg) >> 1; step > 0; step >>= 1) pos = tab[pos + step - 1] < key ? pos + step : pos;
+    const int32_t e = tab[pos];
+    return (e >> 8) == x ? (uint32_t)e & 0xffu : 0u;
+}
+
+// GP_UNROLL flagged lookups interleaved step by step
+__device__ __forceinline__ void staged_flags_n(const int32_t* tab, int lg, const int32_t* x, uint32_t* fl) {
+    int32_t key[GP_UNROLL], pos[GP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) {
+        key[u] = x[u] < 0 ? INT_MIN : (int32_t)((uint32_t)x[u] << 8);
+        pos[u] = 0;
+    }
+    for (int32_t step = (1 << lg) >> 1; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < GP_UNROLL; ++u) pos[u] = tab[pos[u] + step - 1] < key[u] ? pos[u] + step : pos[u];
+    }
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) {
+        const int32_t e = tab[pos[u]];
+        fl[u] = (e >> 8) == x[u] ? (uint32_t)e & 0xffu : 0u;
+    }
+}
+
 // FAITHFUL: replay the reference's intersections and tally B_alg.
 // PARTS: run the parts of deferred heavy tasks instead of the task list.
 // MULTI: several plans share the launch (one plan when false).
-template <bool FAITHFUL, bool PARTS, bool MULTI>
+// EXT: the extended paths, the batched level's flagged shared search
+// (KPlan::bl_base) and the pair level grouped by v1 (KPlan::pair_v1); kept
+// out of the other instantiations, whose register allocation they would
+// otherwise burden.
+template <bool FAITHFUL, bool PARTS, bool MULTI, bool EXT>
 __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
     __shared__ int32_t s_len[GP_MAX_PREFIX];
@@ -501,7 +542,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     // LDS hash, and the candidates' neighbour lists are streamed flattened
     // across the 64 lanes.  Candidates whose list is far longer than the
     // staged sets are handled by probing the sets into the list instead.
-    auto leaf_level = [&](int d) {
+    auto leaf_level = [&](int d) __attribute__((always_inline)) {
         const int32_t cstart = d == 2 ? dstart : 0, cstep = d == 2 ? dstride : 1;
         const int32_t ncand = s_ll[d] > cstart ? (s_ll[d] - cstart + cstep - 1) / cstep : 0;
         if (ncand <= 0) return;
@@ -550,9 +591,46 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 ++nstream;
                 stream_total += gn;
                 if (gn > 0) smax = max(smax, GL(g)[gn - 1]);
+            }
+        }
+        // stage the kind-1 sets: the base with membership flags when it
+        // fits (one search per streamed element answers every kind-1
+        // block), otherwise each set on its own while the pool lasts
+        bool flagged = false;
+        if (EXT && pl->bl_base >= 0 && v_cnt < GP_FLAG_VMAX) {
+            const int bb = pl->bl_base;
+            const int32_t gn = s_bn[bb];
+            const int lg = ilog2_ceil(gn + 1);
+            if (gn > 0 && (1 << lg) <= a.hash_slots) {
+                const gcptr src = GL(s_bg[bb]);
+                for (int32_t i = lane; i < (1 << lg); i += WAVE) {
+                    int32_t e = INT_MAX;
+                    if (i < gn) {
+                        const int32_t y = src[i];
+                        uint32_t f = 0;
+                        for (int b = 0; b < nb; ++b) {
+                            if (pl->bl_kind[b] != 1) continue;
+                            if (b == bb || contains(s_bg[b], s_bn[b], y)) f |= 1u << pl->bl_bit[b];
+                        }
+                        e = (int32_t)((uint32_t)y << 8 | f);
+                    }
+                    s_hash[i] = e;
+                }
+                __syncthreads();
+                // s_bh[bb] stays -1: the probe path reads plain ids, so it
+                // takes the base set from global memory
+                s_blg[bb] = lg;
+                stream_cost_w = 1 + lg / 4;
+                flagged = true;
+            }
+        }
+        if (!flagged) {
+            for (int b = 0; b < nb; ++b) {
+                if (pl->bl_kind[b] != 1) continue;
+                const int32_t gn = s_bn[b];
                 const int lg = ilog2_ceil(gn + 1);
                 if (gn > 0 && hoff + (1 << lg) <= a.hash_slots) {
-                    stage_sorted(s_hash + hoff, lg, g, gn);
+                    stage_sorted(s_hash + hoff, lg, s_bg[b], gn);
                     s_bh[b] = hoff;
                     s_blg[b] = lg;
                     hoff += 1 << lg;
@@ -562,6 +640,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 }
             }
         }
+        const int32_t* const ftab = s_hash;                  // flagged base table
+        const int flg = flagged ? s_blg[pl->bl_base] : 0;
         const bool plain = K == 0;
         const uint32_t leaf_mask = plain ? pl->rmask[L] : 0;
         // 2. candidates in batches of 64
@@ -631,10 +711,13 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 const uint64_t above = lane == WAVE - 1 ? 0ull : starts & (~0ull << (lane + 1));
                 const int next = above ? __ffsll((long long)above) - 1 : WAVE;
                 const uint64_t seg = (next == WAVE ? ~0ull : ((1ull << next) - 1)) & (~0ull << lane);
+                const uint32_t fl = flagged && keep ? staged_flags(ftab, flg, x) : 0u;
                 for (int b2 = 0; b2 < nb; ++b2) {
                     if (pl->bl_kind[b2] != 1) continue;
                     bool hit = false;
-                    if (keep) {
+                    if (flagged) {
+                        hit = fl >> pl->bl_bit[b2] & 1;
+                    } else if (keep) {
                         const int h = s_bh[b2];
                         hit = h >= 0 ? staged_has(s_hash + h, s_blg[b2], x) : contains(s_bg[b2], s_bn[b2], x);
                     }
@@ -656,6 +739,42 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 lm &= lm - 1;
                 const gcptr pv = GL(a.edge) + __shfl(nvs, j);
                 const int32_t pn = __shfl(lwork, j);
+                if (flagged) {
+                    // one pass, one search per element, every kind-1 block at
+                    // once: per-lane counts, two 16-bit fields per register
+                    // (a lane sees at most pn / 64 + 4 elements of a list)
+                    uint32_t pk0 = 0, pk1 = 0, pk2 = 0, pk3 = 0;
+                    const int nk1 = pl->nk1;
+                    for (int32_t f0 = 0; f0 < pn; f0 += GP_UNROLL * WAVE) {
+                        const int32_t f = f0 + lane;
+                        int32_t x[GP_UNROLL];
+#pragma unroll
+                        for (int u = 0; u < GP_UNROLL; ++u) x[u] = f + u * WAVE < pn ? pv[f + u * WAVE] : -1;
+                        uint32_t fl[GP_UNROLL];
+                        staged_flags_n(ftab, flg, x, fl);
+#pragma unroll
+                        for (int u = 0; u < GP_UNROLL; ++u) {
+                            const uint32_t m = fl[u];
+                            pk0 += (m & 1u) | ((m & 2u) << 15);
+                            if (nk1 > 2) pk1 += ((m >> 2) & 1u) | ((m & 8u) << 13);
+                            if (nk1 > 4) pk2 += ((m >> 4) & 1u) | ((m & 32u) << 11);
+                            if (nk1 > 6) pk3 += ((m >> 6) & 1u) | ((m & 128u) << 9);
+                        }
+                    }
+                    for (int b2 = 0; b2 < nb; ++b2) {
+                        if (pl->bl_kind[b2] != 1) continue;
+                        const int q = pl->bl_bit[b2];
+                        const uint32_t w = q < 2 ? pk0 : q < 4 ? pk1 : q < 6 ? pk2 : pk3;
+                        int32_t c = (int32_t)wave_sum_u32((w >> (16 * (q & 1))) & 0xffffu);
+                        const uint32_t em = (uint32_t)s_bemb[b2];
+                        if (em) {
+                            bool sub = lane < d && (em >> lane & 1) && contains((const int32_t*)pv, pn, s_emb[lane]);
+                            c -= popc64(__ballot(sub));
+                        }
+                        if (lane == j) s_bc[b2][j] = c;
+                    }
+                    continue;
+                }
                 for (int b2 = 0; b2 < nb; ++b2) {
                     if (pl->bl_kind[b2] != 1) continue;
                     const int h = s_bh[b2];
@@ -1043,6 +1162,124 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     };
 
     // add this wave's sums for the current plan to its outputs and reset them
+    // Pair level grouped by v1 (KPlan::pair_v1).  The tasks of one fetch
+    // share v1 (edge slots read reversed) with v0 ascending.  The wave's dense
+    // array holds cnt[x] = |{v3 in N(v1) ∩ [0, pv_hi-th element) : x in N(v3)}|
+    // (x != v1), grown as v0 grows; each task streams only its depth-2 lists:
+    //   sum_{a in D} sum_{x in N(a), x < b, x != v0} cnt[x]
+    // minus the pairs v3 == a (a in D ∩ C), which the loop nest skips.  The
+    // array is cleared when the fetch ends, so it is all zero between fetches.
+    int32_t pv_v1 = -1; // v1 whose counts the array holds, -1 = none
+    int32_t pv_hi = 0;  // N(v1)[0:pv_hi) scattered
+    // every list N(u) of the 64 lanes' vertices u (u < 0: none), x != skip:
+    // mode 0 adds 1 at cnt[x], mode 2 writes 0, mode 1 (lists cut at b)
+    // adds cnt[x] to the lane sums
+    auto pv_stream = [&](int mode, int32_t u, int32_t skip, int32_t b) {
+        int32_t* const cnt = a.dense + (int64_t)blockIdx.x * a.dense_stride;
+        const gcptr E = GL(a.edge);
+        int32_t us = 0, un = 0;
+        if (u >= 0) {
+            const uint32_t l = a.vertex[u];
+            us = (int32_t)l;
+            const int32_t dg = (int32_t)(a.vertex[u + 1] - l);
+            un = (mode != 1 || b >= v_cnt || dg <= 16) ? dg : lower_bound_lane(a.edge + l, dg, b);
+        }
+        auto put = [&](int32_t x) {
+            if (x < 0 || x == skip) return;
+            if (mode == 0)
+                atomicAdd(&cnt[x], 1);
+            else if (mode == 2)
+                cnt[x] = 0;
+            else if (x < b)
+                lacc += (uint64_t)__hip_atomic_load(&cnt[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        };
+        const bool lng = un >= GP_LONG_LIST;
+        int32_t tot = 0;
+        const int32_t ex = wave_excl_scan(lng ? 0 : un, &tot);
+        s_off2[lane] = ex;
+        __syncthreads();
+        for (int32_t f0 = 0; f0 < tot; f0 += WAVE) {
+            const int32_t f = f0 + lane;
+            const bool act = f < tot;
+            int lo = 0, hi = WAVE;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off2[mid] <= f) lo = mid; else hi = mid;
+            }
+            const int32_t os = __shfl(us, lo);
+            const int32_t oe = __shfl(ex, lo);
+            put(act ? E[os + (f - oe)] : -1);
+        }
+        uint64_t lm = __ballot(lng);
+        while (lm) {
+            const int j = __ffsll((long long)lm) - 1;
+            lm &= lm - 1;
+            const gcptr pv = E + __shfl(us, j);
+            const int32_t pn = __shfl(un, j);
+            for (int32_t f0 = 0; f0 < pn; f0 += GP_UNROLL * WAVE) {
+                int32_t x[GP_UNROLL];
+#pragma unroll
+                for (int q = 0; q < GP_UNROLL; ++q) x[q] = f0 + q * WAVE + lane < pn ? pv[f0 + q * WAVE + lane] : -1;
+#pragma unroll
+                for (int q = 0; q < GP_UNROLL; ++q) put(x[q]);
+            }
+        }
+        __syncthreads();
+    };
+    // N(pv_v1)[lo:hi): scatter (mode 0) or clear (mode 2) their lists
+    auto pv_range = [&](int mode, int32_t lo, int32_t hi) {
+        const gcptr C = GL(a.edge) + a.vertex[pv_v1];
+        for (int32_t i0 = lo; i0 < hi; i0 += WAVE) {
+            const int32_t i = i0 + lane;
+            pv_stream(mode, i < hi ? C[i] : -1, pv_v1, 0);
+        }
+        __threadfence();
+    };
+    auto pv_clear = [&]() {
+        if (pv_v1 < 0) return;
+        pv_range(2, 0, pv_hi);
+        pv_v1 = -1;
+        pv_hi = 0;
+    };
+    auto pair_level_v1 = [&]() {
+        const int32_t v0 = s_emb[0], v1 = s_emb[1];
+        const uint32_t l1 = a.vertex[v1];
+        const int32_t hi = wave_lower_bound(a.edge + l1, (int32_t)(a.vertex[v1 + 1] - l1), v0);
+        if (pv_v1 != v1 || hi < pv_hi) {
+            pv_clear();
+            pv_v1 = v1;
+            pv_hi = 0;
+        }
+        if (hi > pv_hi) {
+            pv_range(0, pv_hi, hi);
+            pv_hi = hi;
+        }
+        const int32_t* D = s_lp[2];
+        const int32_t Dn = s_ll[2];
+        if (Dn <= 0 || pv_hi <= 0) return;
+        const int32_t b = bound(pl->rmask[L]);
+        const int32_t* C = a.edge + l1;
+        for (int32_t abase = 0; abase < Dn; abase += WAVE) {
+            const int32_t ai = abase + lane;
+            int32_t av = ai < Dn ? GL(D)[ai] : -1;
+            if (av == v0 || av == v1) av = -1;
+            pv_stream(1, av, v0, b);
+            // the pair v3 == av: every kept element of N(av)
+            if (av >= 0 && contains(C, pv_hi, av)) {
+                const uint32_t l = a.vertex[av];
+                const int32_t dg = (int32_t)(a.vertex[av + 1] - l);
+                const gcptr na = GL(a.edge) + l;
+                uint64_t own = 0;
+                for (int32_t i = 0; i < dg; ++i) {
+                    const int32_t x = na[i];
+                    if (x >= b) break;
+                    own += (x != v0 && x != v1) ? 1 : 0;
+                }
+                lacc -= own;
+            }
+        }
+    };
+
     auto flush = [&]() {
         const uint64_t lsum = wave_sum_u64(lacc);
         if (lane == 0) {
@@ -1063,6 +1300,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         uint64_t t;
         if (!PARTS) {
             if (cur >= cend) {
+                if (EXT) pv_clear(); // the dense array is zero between fetches
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
                 base = first_lane_u64(base);
@@ -1111,6 +1349,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         } else {
             v0 = a.src[t];
             v1 = a.edge[t];
+            if (pl->pair_v1) { // the slot read reversed: its row is v1
+                v1 = v0;
+                v0 = a.edge[t];
+            }
             if (v1 == v0) continue;
             if ((pl->rmask[1] & 1) && v1 >= v0) continue;
         }
@@ -1151,7 +1393,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             }
         }
         if (pl->pair && L == 4) {
-            pair_level();
+            if (EXT && !PARTS && pl->pair_v1 && a.dense)
+                pair_level_v1();
+            else
+                pair_level();
             continue;
         }
         if (pl->batched && L == 3) {
@@ -1247,6 +1492,19 @@ struct CallCtx {
     size_t dense_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // graph mode: pinned staging for the plan upload and the read-back, and
+    // the instantiated graph of the last launch shape (see GraphKey)
+    char* h_meta = nullptr;
+    size_t h_meta_cap = 0;
+    unsigned long long* h_out = nullptr; // counters (cnt_cap entries), then 4 clock stamps per wave
+    size_t h_out_cap = 0;                // entries
+    struct GraphEntry {
+        std::vector<char> key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipGraphNode_t count = nullptr, parts = nullptr;
+    };
+    std::vector<GraphEntry> graphs; // most recently used last, at most GP_GRAPH_CACHE
 };
 
 struct gp_graph {
@@ -1279,6 +1537,12 @@ void destroy_ctx(CallCtx* c) {
     if (c->d_heavy) (void)hipFree(c->d_heavy);
     if (c->d_scratch) (void)hipFree(c->d_scratch);
     if (c->d_dense) (void)hipFree(c->d_dense);
+    for (CallCtx::GraphEntry& ge : c->graphs) {
+        if (ge.exec) (void)hipGraphExecDestroy(ge.exec);
+        if (ge.graph) (void)hipGraphDestroy(ge.graph);
+    }
+    if (c->h_meta) (void)hipHostFree(c->h_meta);
+    if (c->h_out) (void)hipHostFree(c->h_out);
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1375,6 +1639,8 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
         int rc = gp_make_kplan(&scheds[p], faithful, !(tuning && tuning->no_batch), !(tuning && tuning->no_pair), &kp);
         if (rc != GP_OK) return rc;
         if (tuning && tuning->no_truncate) std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
+        if (tuning && tuning->no_shared_search) kp.bl_base = -1;
+        if (tuning && tuning->no_pair_v1) kp.pair_v1 = 0;
         // heavy-task splitting only where a depth-2 loop exists
         const bool split = !faithful && !kp.root_mode && kp.leaf >= 3 && split_req >= 0;
         kp.split_min = split ? (split_req > 0 ? split_req : kp.pair ? GP_SPLIT_DEFAULT_PAIR : GP_SPLIT_DEFAULT) : INT_MAX;
@@ -1410,9 +1676,12 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     for (int p = 0; p < nplans; ++p) real_tasks += task_cnt[p];
     // tasks per work fetch: 16, fewer when there are under 32 chunks per wave
     // (small graphs: the last chunks then decide the kernel's length)
+    // (64 for a single plan whose pair level is grouped by v1: its counts
+    // carry over between the tasks of one fetch)
+    const uint64_t chunk_max = nplans == 1 && kps[0].pair_v1 ? 64 : 16;
     const int chunk = tuning && tuning->chunk > 0
                           ? tuning->chunk
-                          : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, real_tasks / (uint64_t)(waves * 32)));
+                          : (int)std::max<uint64_t>(1, std::min<uint64_t>(chunk_max, real_tasks / (uint64_t)(waves * 32)));
     // each plan's items start on a chunk boundary, so no fetch spans two plans
     for (int p = 0; p < nplans; ++p)
         task_off[p + 1] = task_off[p] + (task_cnt[p] + (uint64_t)chunk - 1) / (uint64_t)chunk * (uint64_t)chunk;
@@ -1475,6 +1744,169 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     out->dense_stride = dense_stride;
     out->dense_bytes = dense_bytes;
     return GP_OK;
+}
+
+// The count kernel instantiation for a launch (faithful runs have no parts
+// launch and never take the extended paths).
+const void* count_kernel_fn(bool faithful, bool parts, bool multi, bool flags) {
+    if (faithful)
+        return multi ? reinterpret_cast<const void*>(&gp_count_kernel<true, false, true, false>)
+                     : reinterpret_cast<const void*>(&gp_count_kernel<true, false, false, false>);
+    if (parts) {
+        if (multi)
+            return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, true>)
+                         : reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, false>);
+        return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, true>)
+                     : reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, false>);
+    }
+    if (multi)
+        return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, true>)
+                     : reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, false>);
+    return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, true>)
+                 : reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, false>);
+}
+
+// Graph mode of one counting call.  The graph holds, in order: the plan
+// upload from pinned staging (the whole meta buffer), the counter reset (a
+// fill kernel: no memset nodes), the count launch, the deferred-parts launch
+// when there is one, and the read-back of the counters and clock stamps into
+// pinned staging.  It is built once per launch shape (kernel variants, grid,
+// LDS size and the call context's buffers) and replayed; between replays only
+// the two count kernels' arguments change (hipGraphExecKernelNodeSetParams).
+struct GraphKey {
+    const void* kfun;
+    const void* pfun;
+    unsigned grid;
+    size_t lds;
+    void* d_meta;
+    size_t meta_cap;
+    void* d_cnt;
+    size_t cnt_cap;
+    void* d_clocks;
+    size_t n_clk;
+    void* h_meta;
+    void* h_out;
+};
+
+hipError_t graph_run(CallCtx* c, const void* kfun, const void* pfun, unsigned grid, size_t lds, KArgs& a, KArgs& a2,
+                     const std::vector<char>& meta, size_t n_cnt, size_t n_clk, unsigned long long* host_out,
+                     unsigned long long* clocks, float* ms) {
+    hipError_t e = hipSuccess;
+    if (c->h_meta_cap < c->meta_cap) {
+        if (c->h_meta) (void)hipHostFree(c->h_meta);
+        c->h_meta = nullptr;
+        c->h_meta_cap = 0;
+        e = hipHostMalloc((void**)&c->h_meta, c->meta_cap, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        c->h_meta_cap = c->meta_cap;
+    }
+    const size_t cnt_entries = c->cnt_cap / sizeof(unsigned long long);
+    const size_t out_need = cnt_entries + n_clk;
+    if (c->h_out_cap < out_need) {
+        if (c->h_out) (void)hipHostFree(c->h_out);
+        c->h_out = nullptr;
+        c->h_out_cap = 0;
+        e = hipHostMalloc((void**)&c->h_out, out_need * sizeof(unsigned long long), hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        c->h_out_cap = out_need;
+    }
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key)); // compared bytewise, padding included
+    key.kfun = kfun;
+    key.pfun = pfun;
+    key.grid = grid;
+    key.lds = lds;
+    key.d_meta = c->d_meta;
+    key.meta_cap = c->meta_cap;
+    key.d_cnt = c->d_cnt;
+    key.cnt_cap = c->cnt_cap;
+    key.d_clocks = c->d_clocks;
+    key.n_clk = n_clk;
+    key.h_meta = c->h_meta;
+    key.h_out = c->h_out;
+    const char* kb = reinterpret_cast<const char*>(&key);
+    void* kargs[] = {&a};
+    void* pargs[] = {&a2};
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void*>(kfun);
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(WAVE);
+    kp.sharedMemBytes = (unsigned)lds;
+    kp.kernelParams = kargs;
+    kp.extra = nullptr;
+    hipKernelNodeParams pp = kp;
+    pp.func = const_cast<void*>(pfun);
+    pp.kernelParams = pargs;
+    int hit = -1;
+    for (size_t i = 0; i < c->graphs.size(); ++i)
+        if (c->graphs[i].key.size() == sizeof(key) && std::memcmp(c->graphs[i].key.data(), kb, sizeof(key)) == 0)
+            hit = (int)i;
+    if (hit >= 0) {
+        // most recently used goes last
+        std::rotate(c->graphs.begin() + hit, c->graphs.begin() + hit + 1, c->graphs.end());
+        CallCtx::GraphEntry& ge = c->graphs.back();
+        e = hipGraphExecKernelNodeSetParams(ge.exec, ge.count, &kp);
+        if (e == hipSuccess && pfun) e = hipGraphExecKernelNodeSetParams(ge.exec, ge.parts, &pp);
+        if (e != hipSuccess) return e;
+    } else {
+        if (c->graphs.size() >= GP_GRAPH_CACHE) {
+            CallCtx::GraphEntry& old = c->graphs.front();
+            if (old.exec) (void)hipGraphExecDestroy(old.exec);
+            if (old.graph) (void)hipGraphDestroy(old.graph);
+            c->graphs.erase(c->graphs.begin());
+        }
+        hipGraph_t gr = nullptr;
+        e = hipGraphCreate(&gr, 0);
+        if (e != hipSuccess) return e;
+        hipGraphNode_t up = nullptr, fill = nullptr, cn = nullptr, pn = nullptr, r1 = nullptr, r2 = nullptr;
+        e = hipGraphAddMemcpyNode1D(&up, gr, nullptr, 0, c->d_meta, c->h_meta, c->meta_cap, hipMemcpyHostToDevice);
+        unsigned long long* fp = c->d_cnt;
+        int fn = (int)cnt_entries;
+        void* fargs[] = {&fp, &fn};
+        hipKernelNodeParams fpar{};
+        fpar.func = reinterpret_cast<void*>(&fill_u64_kernel);
+        fpar.gridDim = dim3(1);
+        fpar.blockDim = dim3(256);
+        fpar.sharedMemBytes = 0;
+        fpar.kernelParams = fargs;
+        fpar.extra = nullptr;
+        if (e == hipSuccess) e = hipGraphAddKernelNode(&fill, gr, nullptr, 0, &fpar);
+        hipGraphNode_t deps[2] = {up, fill};
+        if (e == hipSuccess) e = hipGraphAddKernelNode(&cn, gr, deps, 2, &kp);
+        hipGraphNode_t last = cn;
+        if (e == hipSuccess && pfun) {
+            e = hipGraphAddKernelNode(&pn, gr, &cn, 1, &pp);
+            last = pn;
+        }
+        if (e == hipSuccess)
+            e = hipGraphAddMemcpyNode1D(&r1, gr, &last, 1, c->h_out, c->d_cnt, c->cnt_cap, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipGraphAddMemcpyNode1D(&r2, gr, &last, 1, c->h_out + cnt_entries, c->d_clocks,
+                                        n_clk * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+        hipGraphExec_t ex = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            return e;
+        }
+        CallCtx::GraphEntry ge;
+        ge.key.assign(kb, kb + sizeof(key));
+        ge.graph = gr;
+        ge.exec = ex;
+        ge.count = cn;
+        ge.parts = pn;
+        c->graphs.push_back(ge);
+    }
+    std::memcpy(c->h_meta, meta.data(), meta.size());
+    e = hipEventRecord(c->e0, c->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(c->graphs.back().exec, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, c->e0, c->e1);
+    if (e != hipSuccess) return e;
+    std::memcpy(host_out, c->h_out, n_cnt * sizeof(unsigned long long));
+    std::memcpy(clocks, c->h_out + cnt_entries, n_clk * sizeof(unsigned long long));
+    return hipSuccess;
 }
 
 // Count nplans schedules over one graph in one persistent launch (plus the
@@ -1565,48 +1997,54 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.clocks = c->d_clocks;
     std::vector<unsigned long long> host_out(n_cnt, 0);
     std::vector<unsigned long long> clocks(4 * (size_t)waves, 0);
-    e = hipMemcpyAsync(c->d_meta, meta.data(), meta_bytes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, (int)n_cnt);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
     const bool parts_launch = any_split && n_tasks > 0;
-    if (e == hipSuccess && n_tasks > 0) {
-        const dim3 grid((unsigned)waves), block(WAVE);
-        const bool multi = nplans > 1;
-        if (faithful) {
-            if (multi)
-                hipLaunchKernelGGL((gp_count_kernel<true, false, true>), grid, block, lds_bytes, c->stream, a);
-            else
-                hipLaunchKernelGGL((gp_count_kernel<true, false, false>), grid, block, lds_bytes, c->stream, a);
-        } else {
-            if (multi)
-                hipLaunchKernelGGL((gp_count_kernel<false, false, true>), grid, block, lds_bytes, c->stream, a);
-            else
-                hipLaunchKernelGGL((gp_count_kernel<false, false, false>), grid, block, lds_bytes, c->stream, a);
-        }
-        e = hipGetLastError();
-        if (e == hipSuccess && parts_launch) {
-            // the deferred tasks, in parts; the kernel reads how many there are
-            KArgs a2 = a;
-            a2.clocks = c->d_clocks + 2 * waves;
-            if (multi)
-                hipLaunchKernelGGL((gp_count_kernel<false, true, true>), grid, block, lds_bytes, c->stream, a2);
-            else
-                hipLaunchKernelGGL((gp_count_kernel<false, true, false>), grid, block, lds_bytes, c->stream, a2);
+    const bool multi = nplans > 1;
+    // the extended kernel when some plan can take the flagged shared search
+    // or the pair level grouped by v1
+    bool flags = false;
+    for (const KPlan& kp : kps)
+        flags = flags || (kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX) || (kp.pair_v1 && dense_bytes);
+    flags = flags && !faithful;
+    float ms = 0;
+    if (tuning && tuning->use_graph && n_tasks > 0) {
+        // the same device work as below, as one replayable graph
+        const void* kfun = count_kernel_fn(faithful, false, multi, flags);
+        const void* pfun = count_kernel_fn(false, true, multi, flags);
+        KArgs a2 = a;
+        a2.clocks = c->d_clocks + 2 * waves;
+        const size_t n_clk = (parts_launch ? 4 : 2) * (size_t)waves;
+        e = graph_run(c, kfun, parts_launch ? pfun : nullptr, (unsigned)waves, lds_bytes, a, a2, meta, n_cnt, n_clk,
+                      host_out.data(), clocks.data(), &ms);
+    } else {
+        e = hipMemcpyAsync(c->d_meta, meta.data(), meta_bytes, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, (int)n_cnt);
             e = hipGetLastError();
         }
+        if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+        if (e == hipSuccess && n_tasks > 0) {
+            const dim3 grid((unsigned)waves), block(WAVE);
+            void* kargs[] = {&a};
+            e = hipLaunchKernel(count_kernel_fn(faithful, false, multi, flags), grid, block, kargs, lds_bytes, c->stream);
+            if (e == hipSuccess && parts_launch) {
+                // the deferred tasks, in parts; the kernel reads how many there are
+                KArgs a2 = a;
+                a2.clocks = c->d_clocks + 2 * waves;
+                void* pargs[] = {&a2};
+                e = hipLaunchKernel(count_kernel_fn(false, true, multi, flags), grid, block, pargs, lds_bytes, c->stream);
+            }
+        }
+        if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(host_out.data(), c->d_cnt, n_cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               c->stream);
+        if (e == hipSuccess && n_tasks > 0)
+            e = hipMemcpyAsync(clocks.data(), c->d_clocks,
+                               sizeof(unsigned long long) * (parts_launch ? 4 : 2) * (size_t)waves,
+                               hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
     }
-    if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(host_out.data(), c->d_cnt, n_cnt * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && n_tasks > 0)
-        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * (parts_launch ? 4 : 2) * (size_t)waves,
-                           hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    float ms = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
     if (e != hipSuccess) {
         // the context may hold a poisoned stream; drop it
         destroy_ctx(c);

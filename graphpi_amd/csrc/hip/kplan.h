@@ -46,6 +46,15 @@ struct KPlan {
     uint8_t bl_nsrc[GP_MAX_BLOCKS];
     int8_t bl_src[GP_MAX_BLOCKS][GP_MAX_IEP_K];
     int8_t bl_slot[GP_MAX_BLOCKS];
+    // kind-1 blocks sharing one search: bl_base is a kind-1 block whose set
+    // contains every other kind-1 block's set (each of its sources is, or is
+    // an ancestor of, a source of every other kind-1 block), or -1.  When the
+    // base set is staged in LDS, entry i holds base[i] << 8 | flags, with bit
+    // bl_bit[b] set when base[i] is also in block b's set; one search then
+    // answers every kind-1 block.  At most 8 kind-1 blocks.
+    int32_t bl_base;
+    int32_t nk1;
+    int8_t bl_bit[GP_MAX_BLOCKS];
     // pair level (plain leaf): depths leaf-2 and leaf-1 evaluated together
     // when the leaf set is f ∩ N(v) with f = g ∩ N(a) built at leaf-2, and
     // the leaf-1 candidates and both bounds do not depend on a.  Then
@@ -53,6 +62,13 @@ struct KPlan {
     // with G(a) = g ∩ N(a), X = [0, bound(leaf)) minus placed vertices.
     int32_t pair;
     int32_t pair_g;   // prefix g, -1 = every vertex
+    // pair level grouped by v1 (leaf = 4, no g): v1 < v0 at depth 1, the
+    // depth-3 loop runs over N(v1) below v0.  Tasks are then read from the
+    // edge slot reversed (v1 = the slot's row), so that the tasks of one
+    // fetch share v1 with v0 ascending; the counts of N(v3) for v3 in
+    // N(v1) ∩ [0, v0) grow incrementally across those tasks and only the
+    // depth-2 side is streamed per task.
+    int32_t pair_v1;
     // set by the launcher: a task whose depth-2 loop has more candidates is
     // deferred and run in parts (INT32_MAX: never)
     int32_t split_min;

@@ -223,6 +223,37 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool 
         }
     }
 
+    // ---- shared search for the kind-1 blocks ----
+    kp->bl_base = -1;
+    kp->nk1 = 0;
+    for (int b = 0; b < GP_MAX_BLOCKS; ++b) kp->bl_bit[b] = -1;
+    if (kp->batched) {
+        for (int b = 0; b < kp->nblocks; ++b)
+            if (kp->bl_kind[b] == 1) kp->bl_bit[b] = (int8_t)kp->nk1++;
+        // is p equal to q or an ancestor of q (so that q's set is inside p's,
+        // truncation included: q is built from p's stored set)?
+        auto covers = [&](int p, int q) {
+            for (int x = q; x >= 0; x = kp->father[x])
+                if (x == p) return true;
+            return false;
+        };
+        if (kp->nk1 >= 2 && kp->nk1 <= 8) {
+            for (int bb = 0; bb < kp->nblocks && kp->bl_base < 0; ++bb) {
+                if (kp->bl_kind[bb] != 1) continue;
+                bool ok = true;
+                for (int b = 0; b < kp->nblocks && ok; ++b) {
+                    if (b == bb || kp->bl_kind[b] != 1) continue;
+                    for (int i = 0; i < kp->bl_nsrc[bb] && ok; ++i) {
+                        bool found = false;
+                        for (int j = 0; j < kp->bl_nsrc[b]; ++j) found = found || covers(kp->bl_src[bb][i], kp->bl_src[b][j]);
+                        ok = found;
+                    }
+                }
+                if (ok) kp->bl_base = bb;
+            }
+        }
+    }
+
     // ---- pair level ----
     kp->pair = 0;
     kp->pair_g = -1;
@@ -237,6 +268,13 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool 
             kp->pair = 1;
             kp->pair_g = kp->father[f];
         }
+    }
+    kp->pair_v1 = 0;
+    if (kp->pair && kp->leaf == 4 && kp->pair_g < 0) {
+        const int c = kp->loop_set[3];
+        if (kp->rmask[1] == 1u && c >= 0 && kp->father[c] < 0 && kp->depth_of[c] == 1 && kp->rmask[3] == 1u &&
+            (kp->rmask[4] & ~3u) == 0)
+            kp->pair_v1 = 1;
     }
     return GP_OK;
 }

@@ -110,6 +110,15 @@ typedef struct gp_tuning {
     int32_t probe_pct;      /* batched level: a long candidate list is probed instead of streamed
                                when probing's modelled cost x probe_pct / 100 is lower;
                                0 = default (100) */
+    int32_t use_graph;      /* 1 = run the call's device work (plan upload, counter reset, the
+                               count and deferred-parts launches, result read-back) as one
+                               hipGraph, kept per call context and replayed with updated
+                               kernel arguments while the launch shape repeats; 0 = stream
+                               launches.  Counts are identical either way */
+    int32_t no_shared_search; /* 1 = batched leaf level: search every block's set on its own
+                                 instead of one flagged search of a containing set */
+    int32_t no_pair_v1;     /* 1 = pair level: stream both sides per task instead of grouping
+                               the tasks of a fetch by v1 */
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */

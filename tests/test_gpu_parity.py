@@ -419,3 +419,63 @@ def test_chung_lu_skewed_graph_matches_oracle():
             tun.split_min = split
             got = dev.count(s, tun)
             assert (got.raw, got.count) == (want["raw"], want["count"]), (name, split)
+
+
+def test_shared_search_matches_per_block_search():
+    """Batched leaf level: one flagged search of a containing set (default)
+    against a search of every block's set (no_shared_search), over every
+    connected 5- and 6-vertex motif, and against the oracle for a subset."""
+    host = gp.HostGraph.rmat(11, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    off = gp.Tuning()
+    off.no_shared_search = 1
+    n = 0
+    for k in (5, 6):
+        for i, adj in enumerate(gp.motifs(k)):
+            s = _sched(host, adj, 1, 1, 1, tri)
+            a, b = dev.count(s), dev.count(s, off)
+            assert a.raw == b.raw, (k, i)
+            if k == 5:
+                want = ob.pattern_matching(host, s, threads=8)
+                assert (a.raw, a.count) == (want["raw"], want["count"]), (k, i)
+            n += 1
+    assert n == 21 + 112
+
+
+@pytest.mark.parametrize("graph", ["rmat", "chunglu"])
+def test_pair_level_grouped_by_v1(graph):
+    """The 5-cycle's pair level with the tasks of a fetch grouped by v1 and
+    counts carried between them (default) against both sides streamed per task
+    (no_pair_v1), the LDS map (pair_map=1) and the oracle.  Chunk sizes 1, 7
+    and 64 start and end fetches mid-row; split_min 16 sends heavy tasks to
+    the parts launch."""
+    if graph == "rmat":
+        host = gp.HostGraph.rmat(12, 8, 0.57, 0.19, 0.19, 2)
+    else:
+        host = gp.HostGraph.chung_lu(6000, 40000, 2.1, 800.0, 3)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    s = _sched(host, gp.PATTERNS["pentagon"], 1, 1, 1, tri)
+    want = ob.pattern_matching(host, s, threads=8)
+    base = gp.Tuning()
+    base.no_pair_v1 = 1
+    ref = dev.count(s, base)
+    assert (ref.raw, ref.count) == (want["raw"], want["count"])
+    for chunk in (0, 1, 7, 64):
+        for split in (0, 16):
+            t = gp.Tuning()
+            t.chunk = chunk
+            t.split_min = split
+            got = dev.count(s, t)
+            assert (got.raw, got.count) == (want["raw"], want["count"]), (chunk, split)
+    m = gp.Tuning()
+    m.pair_map = 1
+    assert dev.count(s, m).raw == want["raw"]
+    # a batch mixing the 5-cycle with other patterns (the dense array is
+    # shared by the plans of a launch)
+    others = [_sched(host, gp.PATTERNS[n], 1, 1, 1, tri) for n in ("house", "6cycle", "4clique")]
+    res, _ = dev.count_batch([s] + others)
+    assert res[0].raw == want["raw"]
+    for o, r in zip(others, res[1:]):
+        assert r.raw == dev.count(o, base).raw
