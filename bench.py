@@ -77,18 +77,22 @@ def cpu_baseline(host, scheds, names, budget_s=20.0):
     v = host.v_cnt
     for name in names:
         s = scheds[name]
-        # start from 1/4096 of the roots and widen towards the time budget
-        stride = max(1, v // 4096)
+        # start from 1/16384 of the roots and widen towards the time budget;
+        # each step takes at most 16 times the roots (cost is uneven over the
+        # roots) and is predicted to take at most 60 s (one oracle call prints
+        # nothing while it runs)
+        stride = max(1, v // 16384)
         t_sample = 0.0
-        for _ in range(6):
+        for _ in range(8):
             t0 = time.perf_counter()
             ob.pattern_matching(host, s, threads=threads, root_begin=stride // 2, root_end=-1, root_stride=stride)
             t_sample = time.perf_counter() - t0
             log(f"cpu sample {name}: stride {stride}, {t_sample:.2f} s")
             if t_sample >= per / 4 or stride == 1:
                 break
-            # at most 16 times the roots per step: cost is uneven over the roots
-            stride = max(1, stride // 16, min(stride // 2, int(stride * max(t_sample, 1e-3) / per)))
+            t_eff = max(t_sample, 1e-3)
+            stride = max(1, stride // 16, int(stride * t_eff / 60.0) + 1,
+                         min(stride // 2, int(stride * t_eff / per)))
         est = t_sample * stride
         total_est += est
         samples.append(f"{name}: roots v = {stride // 2} + {stride}*i ({(v + stride - 1) // stride} roots), "

@@ -985,13 +985,22 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 const int32_t dg = (int32_t)(a.vertex[av + 1] - l);
                 const gcptr na = E + l;
                 uint64_t own = 0;
-                for (int32_t i = 0; i < dg; ++i) {
-                    const int32_t x = na[i];
-                    if (x >= b) break;
-                    bool keep = true;
-                    for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
-                    if (keep && gpf >= 0) keep = contains(gptr, glen, x);
-                    own += keep ? 1 : 0;
+                if (gpf < 0) {
+                    // the elements below b, less the placed vertices among them
+                    const int32_t cut = b >= v_cnt ? dg : lower_bound_lane(a.edge + l, dg, b);
+                    int32_t c = cut;
+                    for (int j = 0; j < da; ++j)
+                        if (s_emb[j] < b && contains(a.edge + l, cut, s_emb[j])) --c;
+                    own = (uint64_t)(int64_t)c;
+                } else {
+                    for (int32_t i = 0; i < dg; ++i) {
+                        const int32_t x = na[i];
+                        if (x >= b) break;
+                        bool keep = true;
+                        for (int j = 0; j < da; ++j) keep = keep && x != s_emb[j];
+                        if (keep) keep = contains(gptr, glen, x);
+                        own += keep ? 1 : 0;
+                    }
                 }
                 lacc -= own;
             }
@@ -1264,18 +1273,17 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             int32_t av = ai < Dn ? GL(D)[ai] : -1;
             if (av == v0 || av == v1) av = -1;
             pv_stream(1, av, v0, b);
-            // the pair v3 == av: every kept element of N(av)
+            // the pair v3 == av: every kept element of N(av), i.e. those
+            // below b other than v0 and v1
             if (av >= 0 && contains(C, pv_hi, av)) {
                 const uint32_t l = a.vertex[av];
                 const int32_t dg = (int32_t)(a.vertex[av + 1] - l);
-                const gcptr na = GL(a.edge) + l;
-                uint64_t own = 0;
-                for (int32_t i = 0; i < dg; ++i) {
-                    const int32_t x = na[i];
-                    if (x >= b) break;
-                    own += (x != v0 && x != v1) ? 1 : 0;
-                }
-                lacc -= own;
+                const int32_t* na = a.edge + l;
+                const int32_t cut = b >= v_cnt ? dg : lower_bound_lane(na, dg, b);
+                int32_t own = cut;
+                if (v0 < b && contains(na, cut, v0)) --own;
+                if (v1 < b && contains(na, cut, v1)) --own;
+                lacc -= (uint64_t)(int64_t)own;
             }
         }
     };
@@ -1644,6 +1652,9 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
         // heavy-task splitting only where a depth-2 loop exists
         const bool split = !faithful && !kp.root_mode && kp.leaf >= 3 && split_req >= 0;
         kp.split_min = split ? (split_req > 0 ? split_req : kp.pair ? GP_SPLIT_DEFAULT_PAIR : GP_SPLIT_DEFAULT) : INT_MAX;
+        // the pair level grouped by v1 keeps its tasks in place by default: a
+        // part would re-stream the whole N(v1) side, which grouping avoids
+        if (kp.pair_v1 && split_req == 0) kp.split_min = INT_MAX;
         any_split = any_split || split;
         any_batched = any_batched || kp.batched;
         any_k = any_k || kp.k;
@@ -1676,12 +1687,14 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     for (int p = 0; p < nplans; ++p) real_tasks += task_cnt[p];
     // tasks per work fetch: 16, fewer when there are under 32 chunks per wave
     // (small graphs: the last chunks then decide the kernel's length)
-    // (64 for a single plan whose pair level is grouped by v1: its counts
-    // carry over between the tasks of one fetch)
-    const uint64_t chunk_max = nplans == 1 && kps[0].pair_v1 ? 64 : 16;
+    // (256 for a single plan whose pair level is grouped by v1: its counts
+    // carry over between the tasks of one fetch; measured on R-MAT s16-s20,
+    // 256 beat 64 and the default by 1.2-6x despite the coarser balance)
+    const bool grouped = nplans == 1 && kps[0].pair_v1;
     const int chunk = tuning && tuning->chunk > 0
                           ? tuning->chunk
-                          : (int)std::max<uint64_t>(1, std::min<uint64_t>(chunk_max, real_tasks / (uint64_t)(waves * 32)));
+                          : grouped ? 256
+                                    : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, real_tasks / (uint64_t)(waves * 32)));
     // each plan's items start on a chunk boundary, so no fetch spans two plans
     for (int p = 0; p < nplans; ++p)
         task_off[p + 1] = task_off[p] + (task_cnt[p] + (uint64_t)chunk - 1) / (uint64_t)chunk * (uint64_t)chunk;
