@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgraphpi_amd.so")
+# GRAPHPI_AMD_LIB points at another build of the library (same ABI)
+LIB_PATH = os.environ.get("GRAPHPI_AMD_LIB") or os.path.join(_HERE, "libgraphpi_amd.so")
 
 GP_MAX_SIZE = 8
 GP_MAX_PREFIX = 28
@@ -103,6 +104,7 @@ class Tuning(ctypes.Structure):
         ("use_graph", ctypes.c_int32),
         ("no_shared_search", ctypes.c_int32),
         ("no_pair_v1", ctypes.c_int32),
+        ("bitmap_ratio", ctypes.c_int32),
     ]
 
 

@@ -57,6 +57,7 @@
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 #define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
+#define GP_BITMAP_RATIO 16 // a block set goes into an LDS bitmap when its words <= this x its sorted table
 #define GP_FLAG_VMAX ((1 << 23) - 1) // flagged LDS tables pack vertex ids below this with 8 flag bits
 #define GP_GRAPH_CACHE 8          // instantiated launch graphs kept per call context
 
@@ -83,6 +84,7 @@ struct KArgs {
     int32_t* dense;          // pair level: per-wave count array over all vertex ids (kept zero), or null
     int64_t dense_stride;
     int32_t probe_pct;    // probing is chosen when its cost x probe_pct / 100 is below streaming's
+    int32_t bitmap_ratio; // bitmap staging when its words <= bitmap_ratio x the sorted table (0: never)
     int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
     int32_t hash_lg;
     int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
@@ -174,6 +176,8 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
         ns = nb;
         nl = na;
     }
+    // hits are below ub: search only that part of the longer list
+    if (ub != INT_MAX && nl > WAVE) nl = wave_lower_bound(Lg, nl, ub);
     const uint64_t below_me = (1ull << lane) - 1;
     int32_t total = 0;
     for (int32_t base = 0; base < ns; base += WAVE) {
@@ -304,6 +308,14 @@ __device__ __forceinline__ uint32_t staged_has_n(const int32_t* tab, int lg, con
     return hits;
 }
 
+// Bitmap of a sorted set in LDS: bit r of word r / 32 is set when lo + r is
+// in the set; bits values are covered.  x outside [lo, lo + bits) is absent.
+__device__ __forceinline__ bool bitmap_has(const int32_t* tab, int32_t lo, uint32_t bits, int32_t x) {
+    const uint32_t r = (uint32_t)(x - lo);
+    const uint32_t w = r < bits ? (uint32_t)tab[r >> 5] : 0u;
+    return (w >> (r & 31)) & 1u;
+}
+
 // Flagged tables: entry = value << 8 | flags, sorted by value, padded with
 // INT_MAX (value 2^23 - 1, never a vertex id below GP_FLAG_VMAX).  A lookup
 // searches x << 8; the result is the flags of x, or 0 when x is absent.
@@ -359,6 +371,9 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     __shared__ int32_t s_bn[GP_MAX_BATCH_BLOCKS];
     __shared__ int32_t s_bh[GP_MAX_BATCH_BLOCKS];  // LDS hash offset, -1 = not staged
     __shared__ int32_t s_blg[GP_MAX_BATCH_BLOCKS];
+    __shared__ int32_t s_bmo[GP_MAX_BATCH_BLOCKS];  // LDS offset of the block's bitmap, -1 = none
+    __shared__ int32_t s_blo[GP_MAX_BATCH_BLOCKS];  // bitmap: smallest element (bit 0)
+    __shared__ uint32_t s_bbits[GP_MAX_BATCH_BLOCKS]; // bitmap: bits covered (largest - smallest + 1)
     __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kinds 0 and 1)
     __shared__ int32_t s_off[WAVE + 1];
     __shared__ int32_t s_off2[WAVE + 1];
@@ -531,7 +546,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         const int q = pl->loop_set[d];
         s_lp[d] = s_ptr[q];
         int32_t len = s_len[q];
-        if (pl->rmask[d]) len = wave_lower_bound(s_ptr[q], len, bound(pl->rmask[d]));
+        if (pl->rmask[d] && !(pl->cut_done >> d & 1)) len = wave_lower_bound(s_ptr[q], len, bound(pl->rmask[d]));
         s_ll[d] = len;
         s_li[d] = d == 2 ? dstart : 0;
     };
@@ -583,8 +598,9 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             s_bn[b] = gn;
             s_bh[b] = -1;
             s_blg[b] = 0;
+            s_bmo[b] = -1;
             if (kind != 2) {
-                bool hit = lane < d && gn > 0 && contains(g, gn, s_emb[lane]);
+                bool hit = lane < d && !(pl->bl_vmask[b] >> lane & 1) && gn > 0 && contains(g, gn, s_emb[lane]);
                 s_bemb[b] = (int32_t)__ballot(hit);
             }
             if (kind == 1) {
@@ -629,6 +645,30 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 if (pl->bl_kind[b] != 1) continue;
                 const int32_t gn = s_bn[b];
                 const int lg = ilog2_ceil(gn + 1);
+                if (gn > 0) {
+                    // a set spanning few ids goes into a bitmap: one LDS read
+                    // per lookup instead of a search
+                    const int32_t lo = GL(s_bg[b])[0], hi = GL(s_bg[b])[gn - 1];
+                    const uint32_t bits = (uint32_t)(hi - lo) + 1u;
+                    const int32_t words = (int32_t)((bits + 31u) >> 5);
+                    if (words <= a.bitmap_ratio << lg && hoff + words <= a.hash_slots) {
+                        int32_t* tab = s_hash + hoff;
+                        for (int32_t i = lane; i < words; i += WAVE) tab[i] = 0;
+                        __syncthreads();
+                        const gcptr src = GL(s_bg[b]);
+                        for (int32_t i = lane; i < gn; i += WAVE) {
+                            const uint32_t r = (uint32_t)(src[i] - lo);
+                            atomicOr(reinterpret_cast<uint32_t*>(&tab[r >> 5]), 1u << (r & 31));
+                        }
+                        __syncthreads();
+                        s_bmo[b] = hoff;
+                        s_blo[b] = lo;
+                        s_bbits[b] = bits;
+                        hoff += words;
+                        stream_cost_w += 1;
+                        continue;
+                    }
+                }
                 if (gn > 0 && hoff + (1 << lg) <= a.hash_slots) {
                     stage_sorted(s_hash + hoff, lg, s_bg[b], gn);
                     s_bh[b] = hoff;
@@ -670,7 +710,9 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             int32_t pcut = 0; // probed lists: elements below scut
             bool probe = false;
             if (valid && nstream > 0 && scut >= 0) {
-                int32_t cut = deg <= 16 ? deg : lower_bound_lane(nv, deg, scut);
+                // a list short enough for the flattened path is streamed
+                // whole (it filters x < scut), saving the dependent search
+                int32_t cut = deg < GP_LONG_LIST ? deg : lower_bound_lane(nv, deg, scut);
                 pcut = cut;
                 // stream the list, or probe the sets into it when cheaper
                 const int64_t stream_cost = (int64_t)cut * stream_cost_w;
@@ -719,7 +761,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                         hit = fl >> pl->bl_bit[b2] & 1;
                     } else if (keep) {
                         const int h = s_bh[b2];
-                        hit = h >= 0 ? staged_has(s_hash + h, s_blg[b2], x) : contains(s_bg[b2], s_bn[b2], x);
+                        const int bmo = s_bmo[b2];
+                        hit = bmo >= 0 ? bitmap_has(s_hash + bmo, s_blo[b2], s_bbits[b2], x)
+                              : h >= 0 ? staged_has(s_hash + h, s_blg[b2], x)
+                                       : contains(s_bg[b2], s_bn[b2], x);
                     }
                     const uint64_t hm = __ballot(hit);
                     if (start) {
@@ -777,10 +822,14 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 }
                 for (int b2 = 0; b2 < nb; ++b2) {
                     if (pl->bl_kind[b2] != 1) continue;
-                    const int h = s_bh[b2];
-                    const int lg = s_blg[b2];
+                    // wave-uniform block parameters, kept in scalar registers
+                    const int h = __builtin_amdgcn_readfirstlane(s_bh[b2]);
+                    const int lg = __builtin_amdgcn_readfirstlane(s_blg[b2]);
+                    const int bmo = __builtin_amdgcn_readfirstlane(s_bmo[b2]);
+                    const int32_t blo = __builtin_amdgcn_readfirstlane(s_blo[b2]);
+                    const uint32_t bbits = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_bbits[b2]);
                     const int32_t* g = s_bg[b2];
-                    const int32_t gn = s_bn[b2];
+                    const int32_t gn = __builtin_amdgcn_readfirstlane(s_bn[b2]);
                     int32_t c = 0;
                     // wave-uniform trip count: every lane takes part in every ballot
                     for (int32_t f0 = 0; f0 < pn; f0 += GP_UNROLL * WAVE) {
@@ -788,7 +837,12 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                         int32_t x[GP_UNROLL];
 #pragma unroll
                         for (int u = 0; u < GP_UNROLL; ++u) x[u] = f + u * WAVE < pn ? pv[f + u * WAVE] : -1;
-                        if (h >= 0) {
+                        if (bmo >= 0) {
+                            // x = -1 (past the end) falls outside the bitmap
+#pragma unroll
+                            for (int u = 0; u < GP_UNROLL; ++u)
+                                c += popc64(__ballot(bitmap_has(s_hash + bmo, blo, bbits, x[u])));
+                        } else if (h >= 0) {
                             // x = -1 (past the end) never matches a staged entry
                             const uint32_t hits = staged_has_n(s_hash + h, lg, x);
 #pragma unroll
@@ -927,7 +981,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 const uint32_t l = a.vertex[u];
                 us = (int32_t)l;
                 const int32_t dg = (int32_t)(a.vertex[u + 1] - l);
-                ucut = (b >= v_cnt || dg <= 16) ? dg : lower_bound_lane(a.edge + l, dg, b);
+                // every mode filters x < b: short lists need no exact cut
+                ucut = (b >= v_cnt || dg < GP_LONG_LIST) ? dg : lower_bound_lane(a.edge + l, dg, b);
             }
             const bool lng = ucut >= GP_LONG_LIST;
             int32_t tot = 0;
@@ -1191,7 +1246,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             const uint32_t l = a.vertex[u];
             us = (int32_t)l;
             const int32_t dg = (int32_t)(a.vertex[u + 1] - l);
-            un = (mode != 1 || b >= v_cnt || dg <= 16) ? dg : lower_bound_lane(a.edge + l, dg, b);
+            un = (mode != 1 || b >= v_cnt || dg < GP_LONG_LIST) ? dg : lower_bound_lane(a.edge + l, dg, b);
         }
         auto put = [&](int32_t x) {
             if (x < 0 || x == skip) return;
@@ -1646,7 +1701,10 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
         KPlan& kp = kps[p];
         int rc = gp_make_kplan(&scheds[p], faithful, !(tuning && tuning->no_batch), !(tuning && tuning->no_pair), &kp);
         if (rc != GP_OK) return rc;
-        if (tuning && tuning->no_truncate) std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
+        if (tuning && tuning->no_truncate) {
+            std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
+            kp.cut_done = 0; // no loop set is cut when it is built
+        }
         if (tuning && tuning->no_shared_search) kp.bl_base = -1;
         if (tuning && tuning->no_pair_v1) kp.pair_v1 = 0;
         // heavy-task splitting only where a depth-2 loop exists
@@ -2005,6 +2063,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.hash_slots = hash_slots;
     a.hash_lg = hash_lg;
     a.probe_pct = tuning && tuning->probe_pct > 0 ? tuning->probe_pct : 100;
+    a.bitmap_ratio = !tuning || tuning->bitmap_ratio == 0 ? GP_BITMAP_RATIO : std::max(0, tuning->bitmap_ratio);
     a.dense = dense_bytes ? c->d_dense : nullptr;
     a.dense_stride = dense_stride;
     a.clocks = c->d_clocks;

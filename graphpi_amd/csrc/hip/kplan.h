@@ -55,6 +55,12 @@ struct KPlan {
     int32_t bl_base;
     int32_t nk1;
     int8_t bl_bit[GP_MAX_BLOCKS];
+    // bit j: block b's set lies inside N(v_j), so the placed vertex v_j is
+    // never in it (no self-loops) and needs no membership test
+    uint8_t bl_vmask[GP_MAX_BLOCKS];
+    // bit d: the loop set at depth d was already cut at exactly the bound of
+    // rmask[d] when it was built, so entering depth d needs no lower_bound
+    uint32_t cut_done;
     // pair level (plain leaf): depths leaf-2 and leaf-1 evaluated together
     // when the leaf set is f ∩ N(v) with f = g ∩ N(a) built at leaf-2, and
     // the leaf-1 candidates and both bounds do not depend on a.  Then

@@ -223,6 +223,31 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool 
         }
     }
 
+    // ---- vertex sets of the prefixes: bit j when the prefix lies in N(v_j) ----
+    {
+        uint32_t vset[GP_MAX_PREFIX] = {0};
+        for (int d = 0; d < n; ++d)
+            for (int i = 0; i < kp->nbuild[d]; ++i) {
+                const int p = kp->build[d][i];
+                vset[p] = (1u << d) | (kp->father[p] >= 0 ? vset[kp->father[p]] : 0u);
+            }
+        for (int b = 0; b < GP_MAX_BLOCKS; ++b) kp->bl_vmask[b] = 0;
+        if (kp->batched)
+            for (int b = 0; b < kp->nblocks; ++b) {
+                uint32_t m = 0;
+                for (int i = 0; i < kp->bl_nsrc[b]; ++i) m |= vset[kp->bl_src[b][i]];
+                kp->bl_vmask[b] = (uint8_t)m;
+            }
+        // a loop set built with the single bound mask its depth uses
+        kp->cut_done = 0;
+        if (!faithful)
+            for (int d = 1; d < n; ++d) {
+                const int q = kp->loop_set[d];
+                if (q >= 0 && kp->rmask[d] && kp->ub_n[q] == 1 && kp->ub_mask[q][0] == kp->rmask[d])
+                    kp->cut_done |= 1u << d;
+            }
+    }
+
     // ---- shared search for the kind-1 blocks ----
     kp->bl_base = -1;
     kp->nk1 = 0;

@@ -119,6 +119,9 @@ typedef struct gp_tuning {
                                  instead of one flagged search of a containing set */
     int32_t no_pair_v1;     /* 1 = pair level: stream both sides per task instead of grouping
                                the tasks of a fetch by v1 */
+    int32_t bitmap_ratio;   /* batched level: a set goes into an LDS bitmap when the bitmap's
+                               words are at most this many times its sorted table's entries;
+                               0 = default (16), < 0 = never */
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */

@@ -479,3 +479,25 @@ def test_pair_level_grouped_by_v1(graph):
     assert res[0].raw == want["raw"]
     for o, r in zip(others, res[1:]):
         assert r.raw == dev.count(o, base).raw
+
+
+def test_bitmap_staging_matches():
+    """Batched level: sets staged as LDS bitmaps (default ratio, and one that
+    takes a bitmap whenever it fits) against sorted staging only
+    (bitmap_ratio < 0) and the oracle."""
+    host = gp.HostGraph.rmat(12, 12, 0.57, 0.19, 0.19, 7)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    off = gp.Tuning()
+    off.bitmap_ratio = -1
+    wide = gp.Tuning()
+    wide.bitmap_ratio = 1 << 20
+    noshare = gp.Tuning()
+    noshare.no_shared_search = 1
+    noshare.bitmap_ratio = 1 << 20
+    for name in ["triangle", "4clique", "5clique", "house", "qg3", "rectangle", "cycle_6_tri", "hourglass"]:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        want = ob.pattern_matching(host, s, threads=8)
+        for t in (None, off, wide, noshare):
+            got = dev.count(s, t)
+            assert (got.raw, got.count) == (want["raw"], want["count"]), name
