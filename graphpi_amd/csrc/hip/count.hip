@@ -57,6 +57,7 @@
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 #define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
+#define GP_SAMPLE_MIN 64   // LDS entries a sampled index needs at least
 #define GP_BITMAP_RATIO 16 // a block set goes into an LDS bitmap when its words <= this x its sorted table
 #define GP_FLAG_VMAX ((1 << 23) - 1) // flagged LDS tables pack vertex ids below this with 8 flag bits
 #define GP_GRAPH_CACHE 8          // instantiated launch graphs kept per call context
@@ -316,6 +317,51 @@ __device__ __forceinline__ bool bitmap_has(const int32_t* tab, int32_t lo, uint3
     return (w >> (r & 31)) & 1u;
 }
 
+// Sampled index of a sorted set g[0:gn) too large for LDS: tab holds
+// g[i << sk] for i < m, padded with INT_MAX to 2^lg (> m) entries.  The
+// number of samples <= x names the only block of 2^sk elements that can
+// hold x, which is then searched in HBM (a cache line or a few, reused by
+// every lookup of the task).  x = -1 is never found.
+__device__ __forceinline__ bool sampled_has(const int32_t* tab, int lg, int sk, const int32_t* gp, int32_t gn, int32_t x) {
+    int32_t pos = 0;
+    for (int32_t step = (1 << lg) >> 1; step > 0; step >>= 1) pos = tab[pos + step - 1] <= x ? pos + step : pos;
+    if (pos == 0) return false;
+    const gcptr g = GL(gp);
+    int32_t q = (pos - 1) << sk;
+    for (int32_t step = (1 << sk) >> 1; step > 0; step >>= 1) {
+        const int32_t i = q + step - 1;
+        q = (i < gn ? g[i] : INT_MAX) < x ? q + step : q;
+    }
+    return q < gn && g[q] == x;
+}
+
+// GP_UNROLL sampled lookups interleaved step by step
+__device__ __forceinline__ uint32_t sampled_has_n(const int32_t* tab, int lg, int sk, const int32_t* gp, int32_t gn,
+                                                  const int32_t* x) {
+    int32_t pos[GP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) pos[u] = 0;
+    for (int32_t step = (1 << lg) >> 1; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < GP_UNROLL; ++u) pos[u] = tab[pos[u] + step - 1] <= x[u] ? pos[u] + step : pos[u];
+    }
+    const gcptr g = GL(gp);
+    int32_t q[GP_UNROLL];
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) q[u] = pos[u] > 0 ? (pos[u] - 1) << sk : 0;
+    for (int32_t step = (1 << sk) >> 1; step > 0; step >>= 1) {
+#pragma unroll
+        for (int u = 0; u < GP_UNROLL; ++u) {
+            const int32_t i = q[u] + step - 1;
+            q[u] = (i < gn ? g[i] : INT_MAX) < x[u] ? q[u] + step : q[u];
+        }
+    }
+    uint32_t hits = 0;
+#pragma unroll
+    for (int u = 0; u < GP_UNROLL; ++u) hits |= (pos[u] > 0 && q[u] < gn && g[q[u]] == x[u] ? 1u : 0u) << u;
+    return hits;
+}
+
 // Flagged tables: entry = value << 8 | flags, sorted by value, padded with
 // INT_MAX (value 2^23 - 1, never a vertex id below GP_FLAG_VMAX).  A lookup
 // searches x << 8; the result is the flags of x, or 0 when x is absent.
@@ -374,6 +420,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     __shared__ int32_t s_bmo[GP_MAX_BATCH_BLOCKS];  // LDS offset of the block's bitmap, -1 = none
     __shared__ int32_t s_blo[GP_MAX_BATCH_BLOCKS];  // bitmap: smallest element (bit 0)
     __shared__ uint32_t s_bbits[GP_MAX_BATCH_BLOCKS]; // bitmap: bits covered (largest - smallest + 1)
+    __shared__ int32_t s_bso[GP_MAX_BATCH_BLOCKS];  // LDS offset of the block's sampled index, -1 = none
+    __shared__ int32_t s_bsk[GP_MAX_BATCH_BLOCKS];  // sampled index: log2 of the sampling stride
     __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kinds 0 and 1)
     __shared__ int32_t s_off[WAVE + 1];
     __shared__ int32_t s_off2[WAVE + 1];
@@ -599,6 +647,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             s_bh[b] = -1;
             s_blg[b] = 0;
             s_bmo[b] = -1;
+            s_bso[b] = -1;
             if (kind != 2) {
                 bool hit = lane < d && !(pl->bl_vmask[b] >> lane & 1) && gn > 0 && contains(g, gn, s_emb[lane]);
                 s_bemb[b] = (int32_t)__ballot(hit);
@@ -641,8 +690,18 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             }
         }
         if (!flagged) {
-            for (int b = 0; b < nb; ++b) {
-                if (pl->bl_kind[b] != 1) continue;
+            // smallest sets first, so that they keep full tables; a set that
+            // no longer fits gets a sampled index (EXT) or is searched in HBM
+            uint32_t todo = 0;
+            for (int b = 0; b < nb; ++b)
+                if (pl->bl_kind[b] == 1) todo |= 1u << b;
+            while (todo) {
+                int b = __ffs(todo) - 1;
+                for (uint32_t r = todo & (todo - 1); r; r &= r - 1) {
+                    const int c = __ffs(r) - 1;
+                    if (s_bn[c] < s_bn[b]) b = c;
+                }
+                todo &= ~(1u << b);
                 const int32_t gn = s_bn[b];
                 const int lg = ilog2_ceil(gn + 1);
                 if (gn > 0) {
@@ -675,9 +734,28 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     s_blg[b] = lg;
                     hoff += 1 << lg;
                     stream_cost_w += 1 + lg / 4;
-                } else {
-                    stream_cost_w += ilog2_ceil(gn + 1) + 1;
+                    continue;
                 }
+                if (EXT && gn > 0 && a.hash_slots - hoff >= GP_SAMPLE_MIN) {
+                    // sampled index: every 2^sk-th element, the last sample
+                    // <= x names the block of 2^sk elements that may hold x
+                    int32_t room = GP_SAMPLE_MIN;
+                    while (room * 2 <= a.hash_slots - hoff) room *= 2;
+                    const int sk = ilog2_ceil((gn + room - 2) / (room - 1));
+                    const int32_t m = (gn + (1 << sk) - 1) >> sk; // samples, m + 1 <= room
+                    const int slg = ilog2_ceil(m + 1);
+                    int32_t* tab = s_hash + hoff;
+                    const gcptr src = GL(s_bg[b]);
+                    for (int32_t i = lane; i < (1 << slg); i += WAVE) tab[i] = i < m ? src[(int64_t)i << sk] : INT_MAX;
+                    __syncthreads();
+                    s_bso[b] = hoff;
+                    s_blg[b] = slg;
+                    s_bsk[b] = sk;
+                    hoff += 1 << slg;
+                    stream_cost_w += 1 + slg / 4 + sk + 1;
+                    continue;
+                }
+                stream_cost_w += lg + 1;
             }
         }
         const int32_t* const ftab = s_hash;                  // flagged base table
@@ -762,9 +840,11 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     } else if (keep) {
                         const int h = s_bh[b2];
                         const int bmo = s_bmo[b2];
-                        hit = bmo >= 0 ? bitmap_has(s_hash + bmo, s_blo[b2], s_bbits[b2], x)
-                              : h >= 0 ? staged_has(s_hash + h, s_blg[b2], x)
-                                       : contains(s_bg[b2], s_bn[b2], x);
+                        const int so = EXT ? s_bso[b2] : -1;
+                        hit = bmo >= 0  ? bitmap_has(s_hash + bmo, s_blo[b2], s_bbits[b2], x)
+                              : h >= 0  ? staged_has(s_hash + h, s_blg[b2], x)
+                              : so >= 0 ? sampled_has(s_hash + so, s_blg[b2], s_bsk[b2], s_bg[b2], s_bn[b2], x)
+                                        : contains(s_bg[b2], s_bn[b2], x);
                     }
                     const uint64_t hm = __ballot(hit);
                     if (start) {
@@ -828,6 +908,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     const int bmo = __builtin_amdgcn_readfirstlane(s_bmo[b2]);
                     const int32_t blo = __builtin_amdgcn_readfirstlane(s_blo[b2]);
                     const uint32_t bbits = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_bbits[b2]);
+                    const int so = EXT ? __builtin_amdgcn_readfirstlane(s_bso[b2]) : -1;
                     const int32_t* g = s_bg[b2];
                     const int32_t gn = __builtin_amdgcn_readfirstlane(s_bn[b2]);
                     int32_t c = 0;
@@ -845,6 +926,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                         } else if (h >= 0) {
                             // x = -1 (past the end) never matches a staged entry
                             const uint32_t hits = staged_has_n(s_hash + h, lg, x);
+#pragma unroll
+                            for (int u = 0; u < GP_UNROLL; ++u) c += popc64(__ballot(hits >> u & 1));
+                        } else if (EXT && so >= 0) {
+                            const uint32_t hits = sampled_has_n(s_hash + so, lg, __builtin_amdgcn_readfirstlane(s_bsk[b2]), g, gn, x);
 #pragma unroll
                             for (int u = 0; u < GP_UNROLL; ++u) c += popc64(__ballot(hits >> u & 1));
                         } else {

@@ -501,3 +501,22 @@ def test_bitmap_staging_matches():
         for t in (None, off, wide, noshare):
             got = dev.count(s, t)
             assert (got.raw, got.count) == (want["raw"], want["count"]), name
+
+
+def test_sampled_index_matches():
+    """Batched level: block sets too large for the LDS pool get a sampled
+    index (every 2^k-th element in LDS, the rest searched in HBM).  A small
+    pool (lds_stage) on a hub-heavy graph forces it; counts equal the
+    default run and the oracle."""
+    host = gp.HostGraph.chung_lu(8000, 60000, 2.1, 1500.0, 5)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    for stage in (32, 100):
+        small = gp.Tuning()
+        small.lds_stage = stage
+        for name in ["house", "qg3", "cycle_6_tri", "hourglass", "4clique"]:
+            s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+            want = ob.pattern_matching(host, s, threads=8)
+            got = dev.count(s, small)
+            assert (got.raw, got.count) == (want["raw"], want["count"]), (name, stage)
+            assert dev.count(s).raw == want["raw"], name
