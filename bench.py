@@ -150,7 +150,7 @@ def batched_bench(gp, scale=10):
             "speedup": round(loop_wall / batch_wall, 3), "exact": exact}
 
 
-def served_bench(gp, scales=(8, 9, 10, 11), rates=(50, 200, 800), n_req=120, seed=12):
+def served_bench(gp, scales=(8, 9, 10, 11), rates=(50, 200, 800), n_req=120, seed=12, use_graph=0):
     """Counting requests against resident graphs (R-MAT edge factor 16, seeds
     5.., one per scale), random connected 3-6-vertex patterns (spanning tree +
     Bernoulli(0.3) extra edges), Poisson arrivals at each rate: latency p50/p99
@@ -163,7 +163,9 @@ def served_bench(gp, scales=(8, 9, 10, 11), rates=(50, 200, 800), n_req=120, see
     out = {"workload": "R-MAT scales " + "/".join(str(x) for x in scales) +
                        ", edge factor 16, seeds 5-8; random connected 3-6-vertex patterns; Poisson arrivals",
            "requests_per_rate": n_req, "rates": {}}
-    with CountServer(graphs, slots=4, max_batch=32) as srv:
+    tun = gp.Tuning()
+    tun.use_graph = use_graph
+    with CountServer(graphs, slots=4, max_batch=32, tuning=tun) as srv:
         alone = {nm: srv._dev[nm] for nm in names}
         for rate in rates:
             rng = np.random.default_rng(seed + rate)

@@ -1653,6 +1653,7 @@ struct CallCtx {
         hipGraphNode_t count = nullptr, parts = nullptr;
     };
     std::vector<GraphEntry> graphs; // most recently used last, at most GP_GRAPH_CACHE
+    std::vector<std::vector<char> > seen; // launch shapes seen once (not yet instantiated)
 };
 
 struct gp_graph {
@@ -1791,7 +1792,8 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
             kp.cut_done = 0; // no loop set is cut when it is built
         }
         if (tuning && tuning->no_shared_search) kp.bl_base = -1;
-        if (tuning && tuning->no_pair_v1) kp.pair_v1 = 0;
+        // grouping by v1 needs the single-plan launch's 256-task fetches
+        if ((tuning && tuning->no_pair_v1) || nplans > 1) kp.pair_v1 = 0;
         // heavy-task splitting only where a depth-2 loop exists
         const bool split = !faithful && !kp.root_mode && kp.leaf >= 3 && split_req >= 0;
         kp.split_min = split ? (split_req > 0 ? split_req : kp.pair ? GP_SPLIT_DEFAULT_PAIR : GP_SPLIT_DEFAULT) : INT_MAX;
@@ -1843,7 +1845,10 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
         task_off[p + 1] = task_off[p] + (task_cnt[p] + (uint64_t)chunk - 1) / (uint64_t)chunk * (uint64_t)chunk;
     const uint64_t n_tasks = task_off[nplans];
     if (waves < 1) waves = 1;
-    if ((uint64_t)waves > n_tasks / (uint64_t)chunk + 1) waves = (int64_t)(n_tasks / (uint64_t)chunk + 1);
+    // (graph mode keeps the full grid, so that launch shapes, and with them
+    // the cached graphs, stay the same whatever the task count)
+    if (!(tuning && tuning->use_graph) && (uint64_t)waves > n_tasks / (uint64_t)chunk + 1)
+        waves = (int64_t)(n_tasks / (uint64_t)chunk + 1);
     // scratch per wave: one slot per materialised prefix (a subset of one
     // neighbour list of a vertex other than the root, so at most the
     // second-largest degree), two IEP temporaries in faithful mode, and the
@@ -1908,16 +1913,13 @@ const void* count_kernel_fn(bool faithful, bool parts, bool multi, bool flags) {
     if (faithful)
         return multi ? reinterpret_cast<const void*>(&gp_count_kernel<true, false, true, false>)
                      : reinterpret_cast<const void*>(&gp_count_kernel<true, false, false, false>);
+    // multi-plan launches have no extended instantiation
     if (parts) {
-        if (multi)
-            return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, true>)
-                         : reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, false>);
+        if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, false>);
         return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, true>)
                      : reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, false>);
     }
-    if (multi)
-        return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, true>)
-                     : reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, false>);
+    if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, false>);
     return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, true>)
                  : reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, false>);
 }
@@ -1944,9 +1946,12 @@ struct GraphKey {
     void* h_out;
 };
 
+// A shape is instantiated the second time it is seen: one-off shapes run as
+// stream launches (*launched = false) and pay no instantiation.
 hipError_t graph_run(CallCtx* c, const void* kfun, const void* pfun, unsigned grid, size_t lds, KArgs& a, KArgs& a2,
                      const std::vector<char>& meta, size_t n_cnt, size_t n_clk, unsigned long long* host_out,
-                     unsigned long long* clocks, float* ms) {
+                     unsigned long long* clocks, float* ms, bool* launched) {
+    *launched = false;
     hipError_t e = hipSuccess;
     if (c->h_meta_cap < c->meta_cap) {
         if (c->h_meta) (void)hipHostFree(c->h_meta);
@@ -2005,6 +2010,14 @@ hipError_t graph_run(CallCtx* c, const void* kfun, const void* pfun, unsigned gr
         if (e == hipSuccess && pfun) e = hipGraphExecKernelNodeSetParams(ge.exec, ge.parts, &pp);
         if (e != hipSuccess) return e;
     } else {
+        bool again = false;
+        for (const std::vector<char>& k : c->seen)
+            if (k.size() == sizeof(key) && std::memcmp(k.data(), kb, sizeof(key)) == 0) again = true;
+        if (!again) {
+            if (c->seen.size() >= 4 * GP_GRAPH_CACHE) c->seen.erase(c->seen.begin());
+            c->seen.push_back(std::vector<char>(kb, kb + sizeof(key)));
+            return hipSuccess; // first sighting: the caller launches on the stream
+        }
         if (c->graphs.size() >= GP_GRAPH_CACHE) {
             CallCtx::GraphEntry& old = c->graphs.front();
             if (old.exec) (void)hipGraphExecDestroy(old.exec);
@@ -2054,6 +2067,7 @@ hipError_t graph_run(CallCtx* c, const void* kfun, const void* pfun, unsigned gr
         c->graphs.push_back(ge);
     }
     std::memcpy(c->h_meta, meta.data(), meta.size());
+    *launched = true;
     e = hipEventRecord(c->e0, c->stream);
     if (e == hipSuccess) e = hipGraphLaunch(c->graphs.back().exec, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
@@ -2161,8 +2175,12 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     bool flags = false;
     for (const KPlan& kp : kps)
         flags = flags || (kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX) || (kp.pair_v1 && dense_bytes);
-    flags = flags && !faithful;
+    // multi-plan launches stay on the plain instantiations: the extended
+    // multi-plan kernel spills far more (128 B/lane) than the shared search
+    // saves in a batch of mostly cheap patterns
+    flags = flags && !faithful && !multi;
     float ms = 0;
+    bool launched = false;
     if (tuning && tuning->use_graph && n_tasks > 0) {
         // the same device work as below, as one replayable graph
         const void* kfun = count_kernel_fn(faithful, false, multi, flags);
@@ -2171,8 +2189,9 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         a2.clocks = c->d_clocks + 2 * waves;
         const size_t n_clk = (parts_launch ? 4 : 2) * (size_t)waves;
         e = graph_run(c, kfun, parts_launch ? pfun : nullptr, (unsigned)waves, lds_bytes, a, a2, meta, n_cnt, n_clk,
-                      host_out.data(), clocks.data(), &ms);
-    } else {
+                      host_out.data(), clocks.data(), &ms, &launched);
+    }
+    if (e == hipSuccess && !launched) {
         e = hipMemcpyAsync(c->d_meta, meta.data(), meta_bytes, hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, (int)n_cnt);
