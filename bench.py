@@ -13,7 +13,7 @@ launched as consecutive task ranges (the raw sums add up exactly).  The CSR
 is uploaded and the schedules are built before the timed region.  Prints one
 JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2|cfg1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config rmat22|rmat13|chunglu3m]
                     [--patterns a,b] [--no-cpu] [--no-balg]
 """
 import argparse
@@ -29,20 +29,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    "cfg2": {
+    "rmat22": {
         "workload": "R-MAT scale 22, edge factor 8, (0.57,0.19,0.19,0.05), seed 2; pm=1 rt=1 IEP on",
         "gen": ("rmat", 22, 8, 2),
         "patterns": ["4clique"],
         "named": ["4clique", "pentagon", "house"],
     },
-    "cfg3": {
+    "chunglu3m": {
         "workload": "Chung-Lu 3,000,000 vertices, Pareto exponent 2.1, 117M sampled edges, max expected degree "
                     "30000, seed 3; pm=1 rt=1 IEP on",
         "gen": ("chunglu", 3000000, 117000000, 3),
         "patterns": ["4clique"],
         "named": ["4clique", "5clique", "tri_edge_tri"],
     },
-    "cfg1": {
+    "rmat13": {
         "workload": "R-MAT scale 13, edge factor 12, (0.57,0.19,0.19,0.05), seed 1; pm=1 rt=1 IEP on",
         "gen": ("rmat", 13, 12, 1),
         "patterns": ["triangle", "rectangle", "house"],
@@ -203,7 +203,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="rmat22", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-balg", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-pattern batched measurement")

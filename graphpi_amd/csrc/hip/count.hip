@@ -1832,13 +1832,14 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     for (int p = 0; p < nplans; ++p) real_tasks += task_cnt[p];
     // tasks per work fetch: 16, fewer when there are under 32 chunks per wave
     // (small graphs: the last chunks then decide the kernel's length)
-    // (256 for a single plan whose pair level is grouped by v1: its counts
+    // (512 for a single plan whose pair level is grouped by v1: its counts
     // carry over between the tasks of one fetch; measured on R-MAT s16-s20,
-    // 256 beat 64 and the default by 1.2-6x despite the coarser balance)
+    // 256 beat 64 and the default by 1.2-6x despite the coarser balance, and
+    // 512 beat 256 by 6 % at s20)
     const bool grouped = nplans == 1 && kps[0].pair_v1;
     const int chunk = tuning && tuning->chunk > 0
                           ? tuning->chunk
-                          : grouped ? 256
+                          : grouped ? 512
                                     : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, real_tasks / (uint64_t)(waves * 32)));
     // each plan's items start on a chunk boundary, so no fetch spans two plans
     for (int p = 0; p < nplans; ++p)
