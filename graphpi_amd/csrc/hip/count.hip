@@ -162,10 +162,16 @@ __device__ int32_t count_emb_in(const int32_t* __restrict__ a, int32_t n, const 
 
 // A ∩ B restricted to elements < ub.  COUNT: number of such elements not in
 // emb[0:m).  Otherwise the sorted result is written to out (no exclusion).
+__device__ __forceinline__ int ilog2_ceil(int32_t x);
+__device__ void stage_sorted(int32_t* tab, int lg, const int32_t* __restrict__ g, int32_t n);
+__device__ __forceinline__ bool staged_has(const int32_t* tab, int lg, int32_t x);
+
+// pool (LDS, pool_n entries, may be null): when the longer list, cut at ub,
+// fits, it is staged there sorted and searched in LDS instead of HBM.
 template <bool COUNT>
 __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const int32_t* __restrict__ B,
                              int32_t nb, int32_t ub, int32_t* __restrict__ outp, const int32_t* emb,
-                             int m) {
+                             int m, int32_t* pool = nullptr, int32_t pool_n = 0) {
     const int lane = lane_id();
     const gptr out = GLW(outp);
     const int32_t* S = A;
@@ -179,6 +185,14 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
     }
     // hits are below ub: search only that part of the longer list
     if (ub != INT_MAX && nl > WAVE) nl = wave_lower_bound(Lg, nl, ub);
+    int lgp = -1; // staged in the pool: 2^lgp entries
+    if (pool && ns >= 16 && nl >= WAVE) {
+        const int lg = ilog2_ceil(nl + 1);
+        if ((1 << lg) <= pool_n) {
+            stage_sorted(pool, lg, Lg, nl);
+            lgp = lg;
+        }
+    }
     const uint64_t below_me = (1ull << lane) - 1;
     int32_t total = 0;
     for (int32_t base = 0; base < ns; base += WAVE) {
@@ -186,7 +200,7 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
         bool in = i < ns;
         int32_t x = in ? GL(S)[i] : 0;
         bool keep = in && x < ub;
-        bool hit = keep && contains(Lg, nl, x);
+        bool hit = keep && (lgp >= 0 ? staged_has(pool, lgp, x) : contains(Lg, nl, x));
         if (COUNT)
             for (int j = 0; j < m; ++j) hit = hit && x != emb[j];
         uint64_t mk = __ballot(hit);
@@ -474,7 +488,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             const int32_t* fp = s_ptr[f];
             const int32_t fl = s_len[f];
             if (FAITHFUL) balg += 4ull * (uint64_t)((int64_t)fl + deg);
-            len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0);
+            len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0, s_hash, a.hash_slots);
             ptr = out;
             __syncthreads(); // make the new set visible to every lane
         }
