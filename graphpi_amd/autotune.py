@@ -3,8 +3,10 @@
 The counting kernel has a few knobs that trade against each other per graph
 and pattern: persistent waves per CU, tasks per work fetch (root/edge chunk),
 the depth-2 candidate count above which a task is deferred and split into
-parts, and the LDS pool for staged sets (which decides whether a set is
-searched in LDS or by binary search in HBM).  autotune() measures candidate
+parts, the LDS pool for staged sets (which decides whether a set is
+searched in LDS or by binary search in HBM), the bias between probing a
+staged set into a long list and streaming the list, and the size threshold
+for staging a set as an LDS bitmap instead of a sorted table.  autotune() measures candidate
 settings on a fixed sample of the task space (evenly spread task ranges, so
 hub-heavy and light regions are both represented) and keeps the fastest,
 one knob at a time (coordinate descent from the defaults).  Every candidate
@@ -20,10 +22,11 @@ from . import Tuning
 
 KNOBS = {
     "waves_per_cu": [16, 24, 32],
-    "chunk": [0, 1, 4, 16, 64],
+    "chunk": [0, 1, 4, 16, 64, 256, 512],
     "split_min": [0, 128, 512, 4096, -1],
     "lds_stage": [0, 128, 256],
     "probe_pct": [0, 25, 50, 200, 400],
+    "bitmap_ratio": [0, -1, 4, 64],
 }
 
 
