@@ -155,7 +155,8 @@ def served_bench(gp, scales=(8, 9, 10, 11), rates=(50, 200, 800), n_req=120, see
     5.., one per scale), random connected 3-6-vertex patterns (spanning tree +
     Bernoulli(0.3) extra edges), Poisson arrivals at each rate: latency p50/p99
     and throughput of the serving loop, against the same requests served one at
-    a time by plain gp_count calls (a serial queue)."""
+    a time by plain gp_count calls (a serial queue).  use_graph = 1 runs the
+    loop's launches as hipGraphs (default: stream launches)."""
     import numpy as np
     from graphpi_amd.serve import CountServer, Request, percentile, poisson_arrivals, random_pattern, run_load
     graphs = {f"s{sc}": gp.HostGraph.rmat(sc, 16, 0.57, 0.19, 0.19, 5 + i) for i, sc in enumerate(scales)}

@@ -1924,18 +1924,18 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
 
 // The count kernel instantiation for a launch (faithful runs have no parts
 // launch and never take the extended paths).
-const void* count_kernel_fn(bool faithful, bool parts, bool multi, bool flags) {
+const void* count_kernel_fn(bool faithful, bool parts, bool multi, bool ext) {
     if (faithful)
         return multi ? reinterpret_cast<const void*>(&gp_count_kernel<true, false, true, false>)
                      : reinterpret_cast<const void*>(&gp_count_kernel<true, false, false, false>);
     // multi-plan launches have no extended instantiation
     if (parts) {
         if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, false>);
-        return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, true>)
+        return ext ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, true>)
                      : reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, false>);
     }
     if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, false>);
-    return flags ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, true>)
+    return ext ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, true>)
                  : reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, false>);
 }
 
@@ -2187,19 +2187,19 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     const bool multi = nplans > 1;
     // the extended kernel when some plan can take the flagged shared search
     // or the pair level grouped by v1
-    bool flags = false;
+    bool ext = false;
     for (const KPlan& kp : kps)
-        flags = flags || (kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX) || (kp.pair_v1 && dense_bytes);
+        ext = ext || (kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX) || (kp.pair_v1 && dense_bytes);
     // multi-plan launches stay on the plain instantiations: the extended
     // multi-plan kernel spills far more (128 B/lane) than the shared search
     // saves in a batch of mostly cheap patterns
-    flags = flags && !faithful && !multi;
+    ext = ext && !faithful && !multi;
     float ms = 0;
     bool launched = false;
     if (tuning && tuning->use_graph && n_tasks > 0) {
         // the same device work as below, as one replayable graph
-        const void* kfun = count_kernel_fn(faithful, false, multi, flags);
-        const void* pfun = count_kernel_fn(false, true, multi, flags);
+        const void* kfun = count_kernel_fn(faithful, false, multi, ext);
+        const void* pfun = count_kernel_fn(false, true, multi, ext);
         KArgs a2 = a;
         a2.clocks = c->d_clocks + 2 * waves;
         const size_t n_clk = (parts_launch ? 4 : 2) * (size_t)waves;
@@ -2216,13 +2216,13 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         if (e == hipSuccess && n_tasks > 0) {
             const dim3 grid((unsigned)waves), block(WAVE);
             void* kargs[] = {&a};
-            e = hipLaunchKernel(count_kernel_fn(faithful, false, multi, flags), grid, block, kargs, lds_bytes, c->stream);
+            e = hipLaunchKernel(count_kernel_fn(faithful, false, multi, ext), grid, block, kargs, lds_bytes, c->stream);
             if (e == hipSuccess && parts_launch) {
                 // the deferred tasks, in parts; the kernel reads how many there are
                 KArgs a2 = a;
                 a2.clocks = c->d_clocks + 2 * waves;
                 void* pargs[] = {&a2};
-                e = hipLaunchKernel(count_kernel_fn(false, true, multi, flags), grid, block, pargs, lds_bytes, c->stream);
+                e = hipLaunchKernel(count_kernel_fn(false, true, multi, ext), grid, block, pargs, lds_bytes, c->stream);
             }
         }
         if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
