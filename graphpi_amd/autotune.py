@@ -9,9 +9,11 @@ staged set into a long list and streaming the list, and the size threshold
 for staging a set as an LDS bitmap instead of a sorted table.  autotune() measures candidate
 settings on a fixed sample of the task space (evenly spread task ranges, so
 hub-heavy and light regions are both represented) and keeps the fastest,
-one knob at a time (coordinate descent from the defaults).  Every candidate
-must return the same raw sum on the sample; a mismatch raises, since settings
-never change a count.  Results can be cached per graph and schedule.
+one knob at a time (coordinate descent from the defaults).  The result must
+then also beat the defaults on a held-out sample (the ranges shifted by half
+their spacing), or the defaults are kept.  Every candidate must return the
+same raw sum on the sample; a mismatch raises, since settings never change a
+count.  Results can be cached per graph and schedule.
 """
 import ctypes
 import hashlib
@@ -36,11 +38,14 @@ def _copy(t):
     return c
 
 
-def _sample_ranges(total, parts, frac):
+def _sample_ranges(total, parts, frac, phase=0.0):
+    """`parts` ranges covering `frac` of the tasks, spread evenly; `phase`
+    (0..1) shifts every range by that share of the spacing, so a second
+    sample with another phase holds other tasks."""
     width = max(1, int(total * frac / parts))
     out = []
     for i in range(parts):
-        b = total * i // parts
+        b = total * i // parts + int(total * phase / parts)
         e = min(total, b + width)
         if e > b:
             out.append((b, e))
@@ -103,9 +108,25 @@ def autotune(dev, sched, frac=1.0 / 32, parts=8, cache_path=None, log=None):
                 best_ms, cur = ms, cand
         if log:
             log(f"autotune {knob}: {getattr(cur, knob)} ({best_ms:.2f} ms on the sample)")
+    # a held-out sample (other tasks) must confirm the win, or the defaults stay
+    held = None
+    if total > 4096 and best_ms < base_ms:
+        hold = _sample_ranges(total, parts, frac, phase=0.5)
+        d_ms, d_raw = measure(dev, sched, Tuning(), hold, repeats=2)
+        t_ms, t_raw = measure(dev, sched, cur, hold, repeats=2)
+        if t_raw != d_raw:
+            raise RuntimeError(f"tuned settings changed the raw sum on the held-out sample ({t_raw} != {d_raw})")
+        held = {"default_ms": round(d_ms, 3), "tuned_ms": round(t_ms, 3)}
+        if not t_ms < d_ms * 0.97:
+            cur = Tuning()
+            best_ms = base_ms
+        if log:
+            log(f"autotune held-out sample: default {d_ms:.2f} ms, tuned {t_ms:.2f} ms -> "
+                f"{'tuned' if best_ms < base_ms else 'defaults'}")
     knobs = {k: getattr(cur, k) for k in KNOBS}
     report = {"knobs": knobs, "sample_ms_default": round(base_ms, 3), "sample_ms_tuned": round(best_ms, 3),
-              "candidates": tried, "sample_tasks": sum(e - b for b, e in ranges), "tasks": total}
+              "candidates": tried, "sample_tasks": sum(e - b for b, e in ranges), "tasks": total,
+              "held_out": held}
     if cache_path:
         cache[key] = report
         with open(cache_path, "w") as f:
