@@ -122,6 +122,11 @@ class Stats(ctypes.Structure):
     ]
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("leaf", "iep_k", "root_tasks", "batched", "pair", "pair_grouped_v1",
+                                               "shared_search", "blocks", "kind1_blocks", "scratch_slots")]
+
+
 class _Csr(ctypes.Structure):
     _fields_ = [
         ("v_cnt", ctypes.c_int32),
@@ -152,6 +157,7 @@ _SIGS = {
                                       ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]),
     "gp_tc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]),
+    "gp_kernel_plan": (ctypes.c_int, [ctypes.POINTER(SchedFlat), ctypes.POINTER(PlanInfo)]),
     "gp_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
     "gp_last_error": (ctypes.c_char_p, []),
     "gp_schedule_build": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
@@ -351,6 +357,13 @@ def build_schedule(adj, performance_modeling_type=1, restricts_type=1, use_in_ex
                                    1 if use_in_exclusion_optimize else 0, v_cnt, e_cnt, tri_cnt,
                                    ctypes.byref(out), ctypes.byref(valid)))
     return out, bool(valid.value)
+
+
+def kernel_plan(sched):
+    """gp_kernel_plan: which kernel paths the schedule takes, as a dict."""
+    info = PlanInfo()
+    _check(lib().gp_kernel_plan(ctypes.byref(sched), ctypes.byref(info)))
+    return {n: getattr(info, n) for n, _ in PlanInfo._fields_}
 
 
 def motifs(n):

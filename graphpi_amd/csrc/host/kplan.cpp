@@ -303,3 +303,22 @@ int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool 
     }
     return GP_OK;
 }
+
+extern "C" int gp_kernel_plan(const gp_sched_flat* sched, gp_plan_info* out) {
+    if (!sched || !out) return gp_fail(GP_EINVAL, "null argument");
+    KPlan kp;
+    const int rc = gp_make_kplan(sched, false, true, true, &kp);
+    if (rc != GP_OK) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->leaf = kp.leaf;
+    out->iep_k = kp.k;
+    out->root_tasks = kp.root_mode;
+    out->batched = kp.batched;
+    out->pair = kp.pair;
+    out->pair_grouped_v1 = kp.pair_v1;
+    out->shared_search = kp.batched && kp.bl_base >= 0;
+    out->blocks = kp.batched ? kp.nblocks : 0;
+    out->kind1_blocks = kp.batched ? kp.nk1 : 0;
+    out->scratch_slots = kp.nslots;
+    return GP_OK;
+}

@@ -183,6 +183,24 @@ GP_API int gp_count_footprint(const gp_graph* g, const gp_sched_flat* scheds, in
  * gp_tuning.task_begin). */
 GP_API int gp_count_tasks(const gp_graph* g, const gp_sched_flat* sched, uint64_t* tasks);
 
+/* Which kernel paths a schedule takes.  No counterpart in the reference: it
+ * describes how this library evaluates the loop nest of
+ * Graph::pattern_matching (src/graph.cpp:199-398) for the schedule, for
+ * callers who tune or inspect launches.  Host only; needs no device. */
+typedef struct gp_plan_info {
+    int32_t leaf;            /* depth evaluated without a loop */
+    int32_t iep_k;           /* IEP width (0 = plain leaf) */
+    int32_t root_tasks;      /* 1 = tasks are root vertices, 0 = directed edge slots */
+    int32_t batched;         /* the last loop level runs as one batch */
+    int32_t pair;            /* the last two loop levels run together */
+    int32_t pair_grouped_v1; /* the pair level groups a fetch's tasks by v1 (single-schedule launches) */
+    int32_t shared_search;   /* one flagged search answers every kind-1 block */
+    int32_t blocks;          /* leaf blocks of the batched level (1 for a plain leaf) */
+    int32_t kind1_blocks;    /* blocks intersected with each candidate's list */
+    int32_t scratch_slots;   /* prefix sets materialised per wave */
+} gp_plan_info;
+GP_API int gp_kernel_plan(const gp_sched_flat* sched, gp_plan_info* out);
+
 /* Triangle count with each triangle counted once (Graph::triangle_counting_mt,
  * src/graph.cpp:112-139). */
 GP_API int gp_tc(const gp_graph* g, uint64_t* triangles, gp_stats* stats);
