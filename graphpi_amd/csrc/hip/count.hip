@@ -57,6 +57,7 @@
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 #define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
+#define GP_SHARED_MIN_DEG 4096 // the flagged shared search only on graphs with a larger max degree
 #define GP_SAMPLE_MIN 64   // LDS entries a sampled index needs at least
 #define GP_BITMAP_RATIO 16 // a block set goes into an LDS bitmap when its words <= this x its sorted table
 #define GP_FLAG_VMAX ((1 << 23) - 1) // flagged LDS tables pack vertex ids below this with 8 flag bits
@@ -1805,7 +1806,11 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
             std::memset(kp.ub_n, 0, sizeof(kp.ub_n));
             kp.cut_done = 0; // no loop set is cut when it is built
         }
-        if (tuning && tuning->no_shared_search) kp.bl_base = -1;
+        // the flagged shared search pays off only on graphs with large
+        // neighbour sets (measured: slower on R-MAT s10/s12 with max degree
+        // 467/1,319 by 27-35 %, faster on s18 with 15,992); below the
+        // threshold its plans stay on the plain kernel
+        if ((tuning && tuning->no_shared_search) || g->max_deg < GP_SHARED_MIN_DEG) kp.bl_base = -1;
         // grouping by v1 needs the single-plan launch's 256-task fetches
         if ((tuning && tuning->no_pair_v1) || nplans > 1) kp.pair_v1 = 0;
         // heavy-task splitting only where a depth-2 loop exists
