@@ -57,6 +57,9 @@
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 #define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
+#define GP_EXT_NONE 0
+#define GP_EXT_SEARCH 1
+#define GP_EXT_PAIR 2
 #define GP_SHARED_MIN_DEG 4096 // the flagged shared search only on graphs with a larger max degree
 #define GP_SAMPLE_MIN 64   // LDS entries a sampled index needs at least
 #define GP_BITMAP_RATIO 16 // a block set goes into an LDS bitmap when its words <= this x its sorted table
@@ -410,11 +413,14 @@ __device__ __forceinline__ void staged_flags_n(const int32_t* tab, int lg, const
 // FAITHFUL: replay the reference's intersections and tally B_alg.
 // PARTS: run the parts of deferred heavy tasks instead of the task list.
 // MULTI: several plans share the launch (one plan when false).
-// EXT: the extended paths, the batched level's flagged shared search
-// (KPlan::bl_base) and the pair level grouped by v1 (KPlan::pair_v1); kept
-// out of the other instantiations, whose register allocation they would
-// otherwise burden.
-template <bool FAITHFUL, bool PARTS, bool MULTI, bool EXT>
+// EXT: the extended paths, kept out of the other instantiations, whose
+// register allocation they would otherwise burden:
+//   GP_EXT_SEARCH: the batched level's flagged shared search
+//                  (KPlan::bl_base) and sampled indexes of large sets;
+//   GP_EXT_PAIR:   the pair level grouped by v1 (KPlan::pair_v1).
+// A single-schedule launch needs at most one of them (a pair plan has a
+// plain leaf, a flagged plan an IEP leaf).
+template <bool FAITHFUL, bool PARTS, bool MULTI, int EXT>
 __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
     __shared__ int32_t s_len[GP_MAX_PREFIX];
@@ -677,7 +683,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         // fits (one search per streamed element answers every kind-1
         // block), otherwise each set on its own while the pool lasts
         bool flagged = false;
-        if (EXT && pl->bl_base >= 0 && v_cnt < GP_FLAG_VMAX) {
+        if (EXT == GP_EXT_SEARCH && pl->bl_base >= 0 && v_cnt < GP_FLAG_VMAX) {
             const int bb = pl->bl_base;
             const int32_t gn = s_bn[bb];
             const int lg = ilog2_ceil(gn + 1);
@@ -751,7 +757,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     stream_cost_w += 1 + lg / 4;
                     continue;
                 }
-                if (EXT && gn > 0 && a.hash_slots - hoff >= GP_SAMPLE_MIN) {
+                if (EXT == GP_EXT_SEARCH && gn > 0 && a.hash_slots - hoff >= GP_SAMPLE_MIN) {
                     // sampled index: every 2^sk-th element, the last sample
                     // <= x names the block of 2^sk elements that may hold x
                     int32_t room = GP_SAMPLE_MIN;
@@ -855,7 +861,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     } else if (keep) {
                         const int h = s_bh[b2];
                         const int bmo = s_bmo[b2];
-                        const int so = EXT ? s_bso[b2] : -1;
+                        const int so = EXT == GP_EXT_SEARCH ? s_bso[b2] : -1;
                         hit = bmo >= 0  ? bitmap_has(s_hash + bmo, s_blo[b2], s_bbits[b2], x)
                               : h >= 0  ? staged_has(s_hash + h, s_blg[b2], x)
                               : so >= 0 ? sampled_has(s_hash + so, s_blg[b2], s_bsk[b2], s_bg[b2], s_bn[b2], x)
@@ -923,7 +929,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     const int bmo = __builtin_amdgcn_readfirstlane(s_bmo[b2]);
                     const int32_t blo = __builtin_amdgcn_readfirstlane(s_blo[b2]);
                     const uint32_t bbits = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_bbits[b2]);
-                    const int so = EXT ? __builtin_amdgcn_readfirstlane(s_bso[b2]) : -1;
+                    const int so = EXT == GP_EXT_SEARCH ? __builtin_amdgcn_readfirstlane(s_bso[b2]) : -1;
                     const int32_t* g = s_bg[b2];
                     const int32_t gn = __builtin_amdgcn_readfirstlane(s_bn[b2]);
                     int32_t c = 0;
@@ -943,7 +949,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                             const uint32_t hits = staged_has_n(s_hash + h, lg, x);
 #pragma unroll
                             for (int u = 0; u < GP_UNROLL; ++u) c += popc64(__ballot(hits >> u & 1));
-                        } else if (EXT && so >= 0) {
+                        } else if (EXT == GP_EXT_SEARCH && so >= 0) {
                             const uint32_t hits = sampled_has_n(s_hash + so, lg, __builtin_amdgcn_readfirstlane(s_bsk[b2]), g, gn, x);
 #pragma unroll
                             for (int u = 0; u < GP_UNROLL; ++u) c += popc64(__ballot(hits >> u & 1));
@@ -1463,7 +1469,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         uint64_t t;
         if (!PARTS) {
             if (cur >= cend) {
-                if (EXT) pv_clear(); // the dense array is zero between fetches
+                if (EXT == GP_EXT_PAIR) pv_clear(); // the dense array is zero between fetches
                 unsigned long long base = 0;
                 if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
                 base = first_lane_u64(base);
@@ -1556,7 +1562,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             }
         }
         if (pl->pair && L == 4) {
-            if (EXT && !PARTS && pl->pair_v1 && a.dense)
+            if (EXT == GP_EXT_PAIR && !PARTS && pl->pair_v1 && a.dense)
                 pair_level_v1();
             else
                 pair_level();
@@ -1929,19 +1935,21 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
 
 // The count kernel instantiation for a launch (faithful runs have no parts
 // launch and never take the extended paths).
-const void* count_kernel_fn(bool faithful, bool parts, bool multi, bool ext) {
+const void* count_kernel_fn(bool faithful, bool parts, bool multi, int ext) {
     if (faithful)
-        return multi ? reinterpret_cast<const void*>(&gp_count_kernel<true, false, true, false>)
-                     : reinterpret_cast<const void*>(&gp_count_kernel<true, false, false, false>);
+        return multi ? reinterpret_cast<const void*>(&gp_count_kernel<true, false, true, GP_EXT_NONE>)
+                     : reinterpret_cast<const void*>(&gp_count_kernel<true, false, false, GP_EXT_NONE>);
     // multi-plan launches have no extended instantiation
     if (parts) {
-        if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, false>);
-        return ext ? reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, true>)
-                     : reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, false>);
+        if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, GP_EXT_NONE>);
+        if (ext == GP_EXT_SEARCH) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_SEARCH>);
+        if (ext == GP_EXT_PAIR) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_PAIR>);
+        return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_NONE>);
     }
-    if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, false>);
-    return ext ? reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, true>)
-                 : reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, false>);
+    if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, GP_EXT_NONE>);
+    if (ext == GP_EXT_SEARCH) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_SEARCH>);
+    if (ext == GP_EXT_PAIR) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_PAIR>);
+    return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_NONE>);
 }
 
 // Graph mode of one counting call.  The graph holds, in order: the plan
@@ -2190,15 +2198,19 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     std::vector<unsigned long long> clocks(4 * (size_t)waves, 0);
     const bool parts_launch = any_split && n_tasks > 0;
     const bool multi = nplans > 1;
-    // the extended kernel when some plan can take the flagged shared search
-    // or the pair level grouped by v1
-    bool ext = false;
-    for (const KPlan& kp : kps)
-        ext = ext || (kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX) || (kp.pair_v1 && dense_bytes);
-    // multi-plan launches stay on the plain instantiations: the extended
-    // multi-plan kernel spills far more (128 B/lane) than the shared search
-    // saves in a batch of mostly cheap patterns
-    ext = ext && !faithful && !multi;
+    // the extended kernel for a plan that takes the flagged shared search or
+    // the pair level grouped by v1.  Multi-plan launches stay on the plain
+    // instantiations: an extended multi-plan kernel spilled far more
+    // (128 B/lane) than the shared search saved in a batch of mostly cheap
+    // patterns.
+    int ext = GP_EXT_NONE;
+    if (!faithful && !multi) {
+        const KPlan& kp = kps[0];
+        if (kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX)
+            ext = GP_EXT_SEARCH;
+        else if (kp.pair_v1 && dense_bytes)
+            ext = GP_EXT_PAIR;
+    }
     float ms = 0;
     bool launched = false;
     if (tuning && tuning->use_graph && n_tasks > 0) {
