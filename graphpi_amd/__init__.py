@@ -105,6 +105,8 @@ class Tuning(ctypes.Structure):
         ("no_shared_search", ctypes.c_int32),
         ("no_pair_v1", ctypes.c_int32),
         ("bitmap_ratio", ctypes.c_int32),
+        ("slice_ms", ctypes.c_int32),
+        ("time_limit_s", ctypes.c_double),
     ]
 
 
@@ -119,6 +121,10 @@ class Stats(ctypes.Structure):
         ("finish_p90_ms", ctypes.c_double),
         ("finish_max_ms", ctypes.c_double),
         ("split_tasks", ctypes.c_uint64),
+        ("inplace_tasks", ctypes.c_uint64),
+        ("launches", ctypes.c_int32),
+        ("finished", ctypes.c_int32),
+        ("tasks_total", ctypes.c_uint64),
     ]
 
 
@@ -159,6 +165,7 @@ _SIGS = {
     "gp_tc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]),
     "gp_kernel_plan": (ctypes.c_int, [ctypes.POINTER(SchedFlat), ctypes.POINTER(PlanInfo)]),
     "gp_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
+    "gp_device_memory": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     "gp_last_error": (ctypes.c_char_p, []),
     "gp_schedule_build": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
@@ -167,6 +174,7 @@ _SIGS = {
     "gp_csr_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.POINTER(ctypes.POINTER(_Csr))]),
     "gp_csr_complete": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(_Csr))]),
+    "gp_csr_reorder": (ctypes.c_int, [ctypes.POINTER(_Csr), ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(_Csr))]),
     "gp_csr_from_edges": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64,
                                          ctypes.POINTER(ctypes.POINTER(_Csr))]),
     "gp_csr_generate": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
@@ -205,6 +213,13 @@ def device_count():
     n = ctypes.c_int32(0)
     _check(lib().gp_device_count(ctypes.byref(n)))
     return n.value
+
+
+def device_memory():
+    """(free, total) bytes of the current HIP device's memory."""
+    f, t = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(lib().gp_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 # ---------------------------------------------------------------------------
@@ -253,6 +268,13 @@ class HostGraph:
         p = ctypes.POINTER(_Csr)()
         _check(lib().gp_csr_generate(1, n, edges, alpha, max_expected_degree, 0.0, seed, ctypes.byref(p)))
         return cls(p)
+
+    def reorder(self, oriented_type=1):
+        """A relabelled copy: vertices renumbered by degree as the text
+        loader's oriented_type does (1 = high degree first, 2 = low first)."""
+        p = ctypes.POINTER(_Csr)()
+        _check(lib().gp_csr_reorder(self._p, oriented_type, ctypes.byref(p)))
+        return HostGraph(p)
 
     @property
     def v_cnt(self):

@@ -17,5 +17,9 @@ int gp_fail(int code, const std::string& msg);
 int gp_build_csr(std::vector<uint64_t>& pairs, int32_t v_cnt, const std::vector<int64_t>& degree,
                  gp_csr** out);
 
+// O(V+E) check that a CSR is safe to count on (offsets, id range, sorted
+// lists without self-loops); GP_EINVAL with the reason otherwise.
+int gp_csr_validate(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt, uint32_t e_cnt);
+
 // Parallel sort + unique of packed 64-bit keys.
 void gp_sort_unique(std::vector<uint64_t>& keys);

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,7 @@
 #include <vector>
 
 #include "../gp_internal.h"
+#include "../range_plan.h"
 #include "kplan.h"
 #include "schedule.h"
 
@@ -2279,6 +2281,11 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         stats->waves = (int32_t)waves;
         stats->kernel_variant = faithful ? 1 : 0;
         stats->split_tasks = parts_launch ? std::min<uint64_t>(host_out[2], GP_HEAVY_CAP) : 0;
+        stats->inplace_tasks = parts_launch && host_out[2] > GP_HEAVY_CAP ? host_out[2] - GP_HEAVY_CAP : 0;
+        stats->launches = 1;
+        stats->finished = 1;
+        stats->tasks_total = 0;
+        for (int p = 0; p < nplans; ++p) stats->tasks_total += task_cnt[p];
         stats->finish_p50_ms = stats->finish_p90_ms = stats->finish_max_ms = 0;
         if (n_tasks > 0) {
             int rate_khz = 0;
@@ -2312,10 +2319,24 @@ extern "C" int gp_device_count(int32_t* n) {
     return GP_OK;
 }
 
+extern "C" int gp_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return gp_fail(GP_EINVAL, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return gp_fail(GP_ENODEV, "no HIP device");
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return GP_OK;
+}
+
 extern "C" int gp_graph_upload(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt, uint32_t e_cnt,
                                gp_graph** out) {
     if (!out || !vertex || (!edge && e_cnt) || v_cnt < 0) return gp_fail(GP_EINVAL, "bad CSR arguments");
-    if (vertex[v_cnt] != e_cnt) return gp_fail(GP_EINVAL, "vertex[v_cnt] != e_cnt");
+    {
+        const int rc = gp_csr_validate(edge, vertex, v_cnt, e_cnt);
+        if (rc != GP_OK) return rc;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return gp_fail(GP_ENODEV, "no HIP device");
     gp_graph* g = new gp_graph();
@@ -2369,10 +2390,73 @@ extern "C" int gp_graph_bytes(const gp_graph* g, uint64_t* bytes) {
     return GP_OK;
 }
 
+namespace {
+
+double wall_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// One schedule's count as consecutive launches over task ranges
+// (GpRangePlan): no single launch runs for long, and a host-side time limit
+// is checked between launches.  Raw sums of disjoint ranges add up exactly
+// (u64 wrap-around), so the result is the one-launch result.
+int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw_sum,
+                 int64_t* count, gp_stats* stats) {
+    KPlan kp;
+    int rc = gp_make_kplan(sched, false, true, true, &kp);
+    if (rc != GP_OK) return rc;
+    const uint64_t all_tasks = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+    gp_tuning t{};
+    if (tuning) t = *tuning;
+    if (t.task_begin < 0 || (t.task_end > 0 && t.task_end < t.task_begin)) return gp_fail(GP_EINVAL, "bad task range");
+    const uint64_t begin = std::min<uint64_t>((uint64_t)t.task_begin, all_tasks);
+    const uint64_t end = t.task_end > 0 ? std::min<uint64_t>((uint64_t)t.task_end, all_tasks) : all_tasks;
+    GpRangePlan plan(begin, end, t.slice_ms, t.time_limit_s, wall_seconds());
+    if (plan.single()) return run_plans(g, sched, 1, &t, raw_sum, count, stats);
+    gp_stats acc{};
+    uint64_t raw = 0, b = 0, e = 0;
+    while (plan.next(&b, &e)) {
+        gp_tuning sub = t;
+        sub.task_begin = (int64_t)b;
+        sub.task_end = (int64_t)e;
+        uint64_t r = 0;
+        gp_stats st{};
+        rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st);
+        if (rc != GP_OK) return rc;
+        raw += r;
+        const double before_ms = acc.kernel_ms;
+        acc.kernel_ms += st.kernel_ms;
+        acc.balg_bytes += st.balg_bytes;
+        acc.tasks += st.tasks;
+        acc.split_tasks += st.split_tasks;
+        acc.inplace_tasks += st.inplace_tasks;
+        acc.waves = std::max(acc.waves, st.waves);
+        acc.kernel_variant = st.kernel_variant;
+        // finish times of the last launch, offset by the launches before it
+        acc.finish_p50_ms = before_ms + st.finish_p50_ms;
+        acc.finish_p90_ms = before_ms + st.finish_p90_ms;
+        acc.finish_max_ms = before_ms + st.finish_max_ms;
+        plan.finished(e, st.kernel_ms, wall_seconds());
+    }
+    acc.launches = plan.launches;
+    acc.finished = plan.timed_out ? 0 : 1;
+    acc.tasks_total = end > begin ? end - begin : 0;
+    if (raw_sum) *raw_sum = raw;
+    if (stats) *stats = acc;
+    if (plan.timed_out)
+        return gp_fail(GP_ETIME, "time limit of " + std::to_string(t.time_limit_s) + " s reached after " +
+                                     std::to_string(plan.cur - begin) + " of " + std::to_string(end - begin) +
+                                     " tasks");
+    if (count) count[0] = (int64_t)raw / (sched->redundancy ? sched->redundancy : 1);
+    return GP_OK;
+}
+
+} // namespace
+
 extern "C" int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw_sum,
                         int64_t* count, gp_stats* stats) {
     if (!g || !sched) return gp_fail(GP_EINVAL, "null argument");
-    return run_plans(const_cast<gp_graph*>(g), sched, 1, tuning, raw_sum, count, stats);
+    return count_ranged(const_cast<gp_graph*>(g), sched, tuning, raw_sum, count, stats);
 }
 
 extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,

@@ -9,9 +9,12 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <parallel/algorithm>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -64,6 +67,41 @@ int gp_build_csr(std::vector<uint64_t>& pairs, int32_t v_cnt, const std::vector<
     g->max_intersection_size = (int32_t)d2;
     g->tri_cnt = -1;
     *out = g;
+    return GP_OK;
+}
+
+// Checks that a CSR is safe to count on: vertex[0] == 0, offsets
+// non-decreasing and ending at e_cnt, every neighbour id in [0, v_cnt), each
+// list strictly ascending (sorted, no duplicates) and free of self-loops.
+// The kernels binary-search the lists and index by neighbour ids, so a CSR
+// that breaks any of these would read out of bounds or miscount.
+int gp_csr_validate(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt, uint32_t e_cnt) {
+    if (v_cnt < 0) return gp_fail(GP_EINVAL, "negative vertex count");
+    if (!vertex || (!edge && e_cnt)) return gp_fail(GP_EINVAL, "bad CSR arguments");
+    if (vertex[0] != 0) return gp_fail(GP_EINVAL, "vertex[0] != 0");
+    if (vertex[v_cnt] != e_cnt) return gp_fail(GP_EINVAL, "vertex[v_cnt] != e_cnt");
+    int64_t bad_v = -1;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : bad_v)
+    for (int64_t v = 0; v < (int64_t)v_cnt; ++v) {
+        const uint32_t l = vertex[v], r = vertex[v + 1];
+        if (r < l || r > e_cnt) {
+            bad_v = std::max(bad_v, v);
+            continue;
+        }
+        int64_t prev = -1;
+        for (uint32_t i = l; i < r; ++i) {
+            const int64_t x = edge[i];
+            if (x <= prev || x >= v_cnt || x == v) {
+                bad_v = std::max(bad_v, v);
+                break;
+            }
+            prev = x;
+        }
+    }
+    if (bad_v >= 0)
+        return gp_fail(GP_EINVAL, "malformed adjacency of vertex " + std::to_string(bad_v) +
+                                      " (offsets must not decrease, ids must lie in [0, v_cnt), each list "
+                                      "strictly ascending without self-loops)");
     return GP_OK;
 }
 
@@ -151,6 +189,7 @@ extern "C" int gp_csr_load(const char* path, int32_t binary, int32_t oriented_ty
         size_t words = f.size / 4;
         if (words < 3) return gp_fail(GP_EIO, "binary CSR too short");
         uint32_t v_cnt = w[0], e_cnt = w[1];
+        if (v_cnt > (uint32_t)INT32_MAX) return gp_fail(GP_EIO, "binary CSR: vertex count above 2^31-1");
         if (words < 4 + (size_t)v_cnt + e_cnt) return gp_fail(GP_EIO, "binary CSR truncated");
         gp_csr* g = new gp_csr();
         g->v_cnt = (int32_t)v_cnt;
@@ -173,6 +212,11 @@ extern "C" int gp_csr_load(const char* path, int32_t binary, int32_t oriented_ty
         }
         g->max_intersection_size = (int32_t)std::max<int64_t>(d2, 0);
         g->tri_cnt = -1;
+        if (gp_csr_validate(g->edge, g->vertex, (int32_t)v_cnt, e_cnt) != GP_OK) {
+            const std::string why = gp_last_error();
+            gp_csr_free(g);
+            return gp_fail(GP_EIO, "binary CSR: " + why);
+        }
         *out = g;
         return GP_OK;
     }
@@ -180,6 +224,9 @@ extern "C" int gp_csr_load(const char* path, int32_t binary, int32_t oriented_ty
     long long hv, he;
     if (!rd.next(&hv) || !rd.next(&he) || hv < 0 || he < 0)
         return gp_fail(GP_EIO, "missing or malformed header");
+    // v_cnt is an int and e_cnt (2 x the lines) an unsigned int in the CSR
+    if (hv > INT32_MAX || he > (long long)(UINT32_MAX / 2))
+        return gp_fail(GP_EIO, "header counts exceed the CSR's int32 vertex / uint32 edge range");
     const int32_t v_cnt = (int32_t)hv;
     const uint64_t want = (uint64_t)he; // header counts every line, self-loops included
     std::vector<uint64_t> pairs;
@@ -237,6 +284,24 @@ extern "C" int gp_csr_complete(int32_t n, gp_csr** out) {
             ++degree[b];
         }
     return gp_build_csr(pairs, n, degree, out);
+}
+
+extern "C" int gp_csr_reorder(const gp_csr* in, int32_t oriented_type, gp_csr** out) {
+    if (!in || !out) return gp_fail(GP_EINVAL, "null argument");
+    if (oriented_type < 0 || oriented_type > 2) return gp_fail(GP_EINVAL, "oriented_type must be 0, 1 or 2");
+    const int32_t v_cnt = in->v_cnt;
+    std::vector<uint64_t> pairs((size_t)in->e_cnt);
+    std::vector<int64_t> degree((size_t)std::max<int32_t>(v_cnt, 0));
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t v = 0; v < (int64_t)v_cnt; ++v) {
+        degree[v] = (int64_t)in->vertex[v + 1] - in->vertex[v];
+        for (uint32_t i = in->vertex[v]; i < in->vertex[v + 1]; ++i)
+            pairs[i] = (uint64_t)v << 32 | (uint32_t)in->edge[i];
+    }
+    if (oriented_type != 0) reorder_by_degree(pairs, degree, v_cnt, oriented_type);
+    int rc = gp_build_csr(pairs, v_cnt, degree, out);
+    if (rc == GP_OK) (*out)->tri_cnt = in->tri_cnt; // relabelling keeps the triangle count
+    return rc;
 }
 
 extern "C" int gp_csr_from_edges(const int32_t* edges, uint64_t n_edges, gp_csr** out) {

@@ -93,7 +93,16 @@ long long Graph::pattern_matching(const Schedule& schedule, int, bool) {
     rc = upload();
     if (rc != GP_OK) die("gp_graph_upload", rc);
     int64_t count = 0;
-    rc = gp_count(dev_, &flat, &tuning, nullptr, &count, &last_stats);
+    // max_running_time bounds the count as the reference's commented-out
+    // check did (src/graph.cpp:229-237): checked between bounded launches,
+    // and on expiry the process stops with "TIMEOUT!" like its assert(0)
+    gp_tuning t = tuning;
+    if (t.time_limit_s <= 0 && max_running_time > 0) t.time_limit_s = max_running_time;
+    rc = gp_count(dev_, &flat, &t, nullptr, &count, &last_stats);
+    if (rc == GP_ETIME) {
+        std::printf("TIMEOUT!\n");
+        std::fflush(stdout);
+    }
     if (rc != GP_OK) die("gp_count", rc);
     return (long long)count;
 }

@@ -22,7 +22,7 @@ import time
 
 import numpy as np
 
-from . import DeviceGraph, build_schedule
+from . import DeviceGraph, build_schedule, device_memory
 
 
 class Request:
@@ -59,7 +59,11 @@ class CountServer:
     batches may use besides the graphs (a batch larger than the budget still
     runs, alone)."""
 
-    def __init__(self, graphs, slots=4, max_batch=32, mem_budget=32 << 30, tuning=None, device_factory=None):
+    MAX_BATCH = 256  # schedules gp_count_batch takes per call
+
+    def __init__(self, graphs, slots=4, max_batch=32, mem_budget=None, tuning=None, device_factory=None):
+        if not 1 <= max_batch <= self.MAX_BATCH:
+            raise ValueError(f"max_batch must be 1..{self.MAX_BATCH}")
         make = device_factory or DeviceGraph
         self._dev = {}
         self._host = {}
@@ -74,6 +78,11 @@ class CountServer:
         # requests (measured: p50 at 200 q/s 8-10 ms against 2.3-2.6 ms)
         self._tuning = tuning
         self._max_batch = max_batch
+        if mem_budget is None:
+            # most of what is free once the graphs are resident (the rest is
+            # left to the runtime and to other users of the device)
+            free = device_memory()[0] if device_factory is None else 32 << 30
+            mem_budget = int(free * 0.85)
         self._budget = mem_budget
         self._in_flight = 0
         self._pending = collections.deque()
@@ -151,13 +160,22 @@ class CountServer:
         ok = batch
         scheds = [r.sched for r in ok]
         dev = self._dev[graph]
-        fp = dev.footprint(scheds, self._tuning)
-        # admit the longest prefix that fits the free budget (at least one
-        # request when nothing else is in flight)
-        while ok and self._in_flight + fp > self._budget and (self._in_flight > 0 or len(ok) > 1):
-            self._pending.appendleft(ok.pop())
-            scheds.pop()
-            fp = dev.footprint(scheds, self._tuning) if scheds else 0
+        try:
+            fp = dev.footprint(scheds, self._tuning)
+            # admit the longest prefix that fits the free budget (at least one
+            # request when nothing else is in flight)
+            while ok and self._in_flight + fp > self._budget and (self._in_flight > 0 or len(ok) > 1):
+                self._pending.appendleft(ok.pop())
+                scheds.pop()
+                fp = dev.footprint(scheds, self._tuning) if scheds else 0
+        except Exception as e:
+            # the requests taken out of the queue fail; the slot keeps serving
+            now = time.perf_counter()
+            for r in ok:
+                r.error = e
+                r.t_done = now
+                r._event.set()
+            return None
         if not ok:
             return None
         self._in_flight += fp

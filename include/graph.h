@@ -15,7 +15,7 @@ public:
     int v_cnt;          // number of vertices
     unsigned int e_cnt; // number of directed edges (2 x undirected)
     long long tri_cnt;  // number of triangles, -1 when unknown
-    double max_running_time = 60 * 60 * 24; // seconds (kept for interface parity)
+    double max_running_time = 60 * 60 * 24; // seconds; a count still running after this aborts with "TIMEOUT!"
 
     int* edge;             // neighbours, sorted ascending per vertex
     unsigned int* vertex;  // N(v) = edge[vertex[v] .. vertex[v+1])

@@ -39,6 +39,7 @@ extern "C" {
 #define GP_EIO (-4)        /* file missing or malformed */
 #define GP_EPATTERN (-5)   /* pattern has a disconnected prefix (is_pattern_valid == false) */
 #define GP_ENODEV (-6)     /* no HIP device */
+#define GP_ETIME (-7)      /* gp_tuning.time_limit_s ran out before the count finished */
 
 /* ---- limits ---- */
 #define GP_MAX_SIZE 8                       /* pattern vertices */
@@ -122,6 +123,15 @@ typedef struct gp_tuning {
     int32_t bitmap_ratio;   /* batched level: a set goes into an LDS bitmap when the bitmap's
                                words are at most this many times its sorted table's entries;
                                0 = default (16), < 0 = never */
+    int32_t slice_ms;       /* gp_count runs a long count as consecutive launches over task
+                               ranges, each sized from the previous one's rate for about this
+                               many milliseconds of kernel time (a first launch takes 1/256 of
+                               the tasks when there are more than 2^20); 0 = default (2000),
+                               < 0 = one launch */
+    double time_limit_s;    /* gp_count: when > 0, stop between launches once this much wall
+                               time has passed and return GP_ETIME (the reference's
+                               Graph::max_running_time, include/graph.h:12); every launch
+                               still runs to its end.  0 = no limit */
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */
@@ -135,6 +145,10 @@ typedef struct gp_stats {
     double finish_p90_ms;   /* ... 90% of the waves */
     double finish_max_ms;   /* ... every wave (load-balance tail) */
     uint64_t split_tasks;   /* tasks deferred and run in parts */
+    uint64_t inplace_tasks; /* heavy tasks past the deferred list's 2^20 entries, run in place */
+    int32_t launches;       /* count launches (task ranges) the call made */
+    int32_t finished;       /* 1 when every task was counted; 0 after GP_ETIME */
+    uint64_t tasks_total;   /* tasks the call was asked to count */
 } gp_stats;
 
 typedef struct gp_graph gp_graph; /* device-resident CSR (opaque) */
@@ -142,7 +156,10 @@ typedef struct gp_graph gp_graph; /* device-resident CSR (opaque) */
 /* ---- device side ---- */
 
 /* Copy a host CSR (include/graph.h:9-15 layout: vertex[v_cnt+1], edge[e_cnt],
- * e_cnt directed, each list sorted ascending) to the current HIP device. */
+ * e_cnt directed, each list sorted ascending) to the current HIP device.
+ * The CSR is checked first (one pass over it): vertex[0] == 0, offsets
+ * non-decreasing up to vertex[v_cnt] == e_cnt, ids in [0, v_cnt), each list
+ * strictly ascending with no self-loop; anything else is GP_EINVAL. */
 GP_API int gp_graph_upload(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
                            uint32_t e_cnt, gp_graph** out);
 GP_API int gp_graph_free(gp_graph* g);
@@ -154,7 +171,12 @@ GP_API int gp_graph_bytes(const gp_graph* g, uint64_t* bytes);
  * division by sched->redundancy); *count receives (int64)raw_sum / redundancy,
  * which is what Graph::pattern_matching returns (src/graph.cpp:244).  Either
  * pointer may be NULL, as may tuning and stats.  Synchronous on the caller's
- * thread; uses its own HIP stream. */
+ * thread; uses its own HIP stream.  A long count runs as several bounded
+ * launches over consecutive task ranges (gp_tuning.slice_ms); the raw sums of
+ * the ranges add up.  With gp_tuning.time_limit_s set, the call returns
+ * GP_ETIME once the limit has passed between two launches: *raw_sum then
+ * holds the partial sum of the ranges counted, stats->tasks how many tasks
+ * they covered, and *count is left unchanged. */
 GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
                     uint64_t* raw_sum, int64_t* count, gp_stats* stats);
 
@@ -207,6 +229,9 @@ GP_API int gp_tc(const gp_graph* g, uint64_t* triangles, gp_stats* stats);
 
 /* Number of HIP devices visible (0 when none). */
 GP_API int gp_device_count(int32_t* n);
+/* Free and total bytes of the current HIP device's memory (for admission by
+ * footprint; no reference counterpart).  GP_ENODEV without a device. */
+GP_API int gp_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
 
 GP_API const char* gp_last_error(void);
 
@@ -239,6 +264,12 @@ typedef struct gp_csr {
  * degree first.  binary=1 reads the u32 CSR format of
  * DataLoader::twitter_load_data (src/dataloader.cpp:165-190). */
 GP_API int gp_csr_load(const char* path, int32_t binary, int32_t oriented_type, gp_csr** out);
+/* Relabel by degree as the text loader's oriented_type does
+ * (src/dataloader.cpp:96-110): vertices ranked by degree with std::sort on
+ * (id, degree) pairs, 1 = high degree first, 2 = low degree first, 0 = copy.
+ * reorder(load_text(f, 0), t) equals load_text(f, t) for a file f without
+ * duplicate lines (the reference's degrees count duplicates). */
+GP_API int gp_csr_reorder(const gp_csr* in, int32_t oriented_type, gp_csr** out);
 /* Complete graph K_n (DataLoader::load_complete, src/dataloader.cpp:192-262). */
 GP_API int gp_csr_complete(int32_t n, gp_csr** out);
 /* Build from an undirected edge list with the same semantics as writing it as

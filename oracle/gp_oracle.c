@@ -49,6 +49,7 @@ typedef struct {
     int32_t emb[GP_MAX_SIZE];
     int32_t emb_n;
     int32_t* tmp; /* IEP scratch, max_degree entries */
+    int32_t v1_begin, v1_end; /* depth-1 filter (test slicing; the reference has none) */
     uint64_t ans, balg, expanded;
 } ctx_t;
 
@@ -184,6 +185,7 @@ static void level(ctx_t* c, int depth) {
     for (int32_t i = 0; i < loop_size; ++i) {
         int32_t v = loop[i];
         if (mv <= v) break;
+        if (depth == 1 && (v < c->v1_begin || v >= c->v1_end)) continue;
         int dup = 0;
         for (int j = 0; j < c->emb_n; ++j)
             if (c->emb[j] == v) {
@@ -213,7 +215,18 @@ int64_t gpo_pattern_matching(const int32_t* edge, const uint32_t* vertex, int32_
                              const gp_sched_flat* s, int threads, int64_t root_begin,
                              int64_t root_end, int64_t root_stride, uint64_t* raw_sum,
                              uint64_t* balg, uint64_t* expanded) {
+    return gpo_pattern_matching_v1(edge, vertex, v_cnt, s, threads, root_begin, root_end, root_stride, 0,
+                                   -1, raw_sum, balg, expanded);
+}
+
+int64_t gpo_pattern_matching_v1(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
+                                const gp_sched_flat* s, int threads, int64_t root_begin,
+                                int64_t root_end, int64_t root_stride, int64_t v1_begin,
+                                int64_t v1_end, uint64_t* raw_sum, uint64_t* balg,
+                                uint64_t* expanded) {
     if (root_end < 0 || root_end > v_cnt) root_end = v_cnt;
+    if (v1_end < 0 || v1_end > v_cnt) v1_end = v_cnt;
+    if (v1_begin < 0) v1_begin = 0;
     if (root_stride < 1) root_stride = 1;
     /* buffer sizes: second-largest and largest degree (src/dataloader.cpp:110-115) */
     int32_t d1 = 0, d2 = 0;
@@ -238,6 +251,8 @@ int64_t gpo_pattern_matching(const int32_t* edge, const uint32_t* vertex, int32_
         c.vertex = vertex;
         c.v_cnt = v_cnt;
         c.s = s;
+        c.v1_begin = (int32_t)v1_begin;
+        c.v1_end = (int32_t)v1_end;
         for (int p = 0; p < s->total_prefix_num; ++p)
             if (s->father[p] != -1) c.sets[p].buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)d2);
         c.tmp = (int32_t*)malloc(sizeof(int32_t) * (size_t)d1);

@@ -26,6 +26,17 @@ int64_t gpo_pattern_matching(const int32_t* edge, const uint32_t* vertex, int32_
                              int64_t root_end, int64_t root_stride, uint64_t* raw_sum,
                              uint64_t* balg, uint64_t* expanded);
 
+/* The same loop nest with a depth-1 filter, for slicing a count the way the
+ * device slices it: only embeddings whose depth-1 vertex lies in
+ * [v1_begin, v1_end) are counted (v1_end < 0 means v_cnt).  The reference has
+ * no such filter; with the full range this is gpo_pattern_matching.  For a
+ * pattern whose leaf is at depth 1 the filter is never consulted. */
+int64_t gpo_pattern_matching_v1(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
+                                const gp_sched_flat* s, int threads, int64_t root_begin,
+                                int64_t root_end, int64_t root_stride, int64_t v1_begin,
+                                int64_t v1_end, uint64_t* raw_sum, uint64_t* balg,
+                                uint64_t* expanded);
+
 /* Graph::triangle_counting_mt restated (src/graph.cpp:112-139, :66-95). */
 int64_t gpo_triangle_counting(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
                               int threads);

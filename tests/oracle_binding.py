@@ -28,6 +28,12 @@ def olib():
                                            ctypes.POINTER(gp.SchedFlat), ctypes.c_int, ctypes.c_int64,
                                            ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        l.gpo_pattern_matching_v1.restype = ctypes.c_int64
+        l.gpo_pattern_matching_v1.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                              ctypes.POINTER(gp.SchedFlat), ctypes.c_int, ctypes.c_int64,
+                                              ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_uint64)]
         l.gpo_triangle_counting.restype = ctypes.c_int64
         l.gpo_triangle_counting.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int]
         l.gpo_brute_force.restype = ctypes.c_int64
@@ -45,15 +51,16 @@ def _csr(g):
     return v, e
 
 
-def pattern_matching(g, sched, threads=8, root_begin=0, root_end=-1, root_stride=1):
-    """Returns dict(count, raw, balg, expanded)."""
+def pattern_matching(g, sched, threads=8, root_begin=0, root_end=-1, root_stride=1, v1_begin=0, v1_end=-1):
+    """Returns dict(count, raw, balg, expanded).  v1_begin/v1_end keep only
+    embeddings whose depth-1 vertex is in [v1_begin, v1_end) (test slicing)."""
     v, e = _csr(g)
     raw = ctypes.c_uint64()
     balg = ctypes.c_uint64()
     exp = ctypes.c_uint64()
-    c = olib().gpo_pattern_matching(e.ctypes.data, v.ctypes.data, g.v_cnt, ctypes.byref(sched), threads,
-                                    root_begin, root_end, root_stride, ctypes.byref(raw), ctypes.byref(balg),
-                                    ctypes.byref(exp))
+    c = olib().gpo_pattern_matching_v1(e.ctypes.data, v.ctypes.data, g.v_cnt, ctypes.byref(sched), threads,
+                                       root_begin, root_end, root_stride, v1_begin, v1_end, ctypes.byref(raw),
+                                       ctypes.byref(balg), ctypes.byref(exp))
     return {"count": c, "raw": raw.value, "balg": balg.value, "expanded": exp.value}
 
 

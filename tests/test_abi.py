@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import graphpi_amd as gp
@@ -86,3 +87,35 @@ int main(int argc, char** argv) {
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe)], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.split()[1] == "6"
+
+
+@pytest.mark.parametrize("vertex,edge", [
+    ([0, 2, 1, 3], [1, 2, 0]),          # offsets go backwards
+    ([0, 1, 2, 4], [1, 0, 0, 7]),       # neighbour id out of range
+    ([0, 2, 3, 4], [2, 1, 0, 0]),       # unsorted list
+    ([0, 2, 3, 4], [0, 1, 0, 0]),       # self-loop
+    ([0, 2, 3, 5], [1, 2, 0, 0]),       # vertex[v_cnt] != e_cnt
+    ([0, 5, 5, 4], [1, 2, 0, 0]),       # offset beyond e_cnt
+])
+def test_upload_rejects_malformed_csr(vertex, edge):
+    # the check runs before any device call, so it is the same with or without a GPU
+    v = np.asarray(vertex, dtype=np.uint32)
+    e = np.asarray(edge, dtype=np.int32)
+    h = ctypes.c_void_p()
+    rc = gp.lib().gp_graph_upload(e.ctypes.data, v.ctypes.data, 3, 4, ctypes.byref(h))
+    assert rc == -1, gp.lib().gp_last_error()
+    assert not h.value
+
+
+def test_range_plan_cpu(tmp_path):
+    # gp_count's launch plan (task ranges, slice sizing, time limit) with
+    # made-up kernel times: tests/cpp/range_plan_test.cpp
+    exe = tmp_path / "range_plan_test"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                        "-I" + os.path.join(ROOT, "graphpi_amd", "csrc"),
+                        os.path.join(ROOT, "tests", "cpp", "range_plan_test.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    print(out.stdout)
+    assert out.returncode == 0 and out.stdout.strip().endswith("OK"), out.stdout

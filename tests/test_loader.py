@@ -96,3 +96,59 @@ def test_complete_graph():
     h = gp.HostGraph.complete(6)
     assert h.v_cnt == 6 and h.e_cnt == 30
     assert all(np.diff(h.vertex) == 5)
+
+
+def _bin(path, v_cnt, e_cnt, max_deg, vertex, edge):
+    words = [v_cnt, e_cnt, max_deg] + list(vertex) + list(edge)
+    np.asarray(words, dtype=np.uint32).tofile(path)
+    return str(path)
+
+
+@pytest.mark.parametrize("vertex,edge,why", [
+    ([0, 2, 1, 3], [1, 2, 0], "offsets go backwards"),
+    ([0, 1, 2, 4], [1, 0, 0, 7], "neighbour id >= v_cnt"),
+    ([0, 2, 3, 4], [2, 1, 0, 0], "unsorted list"),
+    ([0, 2, 3, 4], [1, 1, 0, 0], "duplicate neighbour"),
+    ([0, 2, 3, 4], [0, 1, 0, 0], "self-loop"),
+    ([1, 2, 3, 4], [1, 0, 0, 0], "vertex[0] != 0"),
+])
+def test_malformed_binary_rejected(tmp_path, vertex, edge, why):
+    p = _bin(tmp_path / "bad.bin", 3, len(edge), 2, vertex, edge)
+    with pytest.raises(gp.GPError) as e:
+        gp.HostGraph.load_binary(p)
+    assert e.value.code == -4, why
+
+
+def test_binary_vertex_count_beyond_int32_rejected(tmp_path):
+    p = str(tmp_path / "huge.bin")
+    np.asarray([0x80000000, 0, 0, 0], dtype=np.uint32).tofile(p)
+    with pytest.raises(gp.GPError) as e:
+        gp.HostGraph.load_binary(p)
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("header", ["3000000000 1", "3 3000000000"])
+def test_text_header_beyond_csr_range_rejected(tmp_path, header):
+    p = _write(tmp_path / "huge.txt", header + "\n1 2\n")
+    with pytest.raises(gp.GPError) as e:
+        gp.HostGraph.load_text(str(p))
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("oriented", [1, 2])
+def test_reorder_equals_loading_with_orientation(tmp_path, oriented):
+    # the in-memory relabelling is the text loader's oriented_type step
+    # (src/dataloader.cpp:96-110): reorder(load(f, 0)) == load(f, oriented)
+    for seed in (3, 4):
+        h = gp.HostGraph.rmat(11, 8, seed=seed)
+        p = str(tmp_path / f"g{seed}.txt")
+        h.write_text(p)
+        plain = gp.HostGraph.load_text(p)
+        want = gp.HostGraph.load_text(p, oriented_type=oriented)
+        got = plain.reorder(oriented)
+        assert (got.v_cnt, got.e_cnt, got.hash()) == (want.v_cnt, want.e_cnt, want.hash())
+        deg = np.diff(got.vertex.astype(np.int64))
+        assert np.all(np.diff(deg) <= 0) if oriented == 1 else np.all(np.diff(deg) >= 0)
+    assert plain.reorder(0).hash() == plain.hash()
+    with pytest.raises(gp.GPError):
+        plain.reorder(3)

@@ -100,3 +100,38 @@ def test_bad_pattern_fails_alone():
             assert False, "a disconnected pattern must fail"
         except ValueError:
             pass
+
+
+def test_footprint_error_fails_requests_and_keeps_serving():
+    # a footprint query that raises fails the requests it covered; the slot
+    # survives and serves the requests that come after
+    log = {"in_flight": 0, "peak": 0, "sizes": []}
+    calls = {"n": 0}
+
+    class Flaky(FakeDevice):
+        def footprint(self, scheds, tuning=None):
+            calls["n"] += 1
+            if calls["n"] == 1:
+                raise gp.GPError(-3, "schedule count out of range")
+            return super().footprint(scheds, tuning)
+
+    graphs = {"a": gp.HostGraph.complete(8)}
+    with CountServer(graphs, slots=1, device_factory=lambda h: Flaky(h, log)) as srv:
+        first = srv.submit(Request(0, "a", gp.PATTERNS["triangle"]))
+        try:
+            first.wait(10)
+            assert False, "the first request must fail"
+        except gp.GPError:
+            pass
+        second = srv.submit(Request(1, "a", gp.PATTERNS["rectangle"]))
+        assert second.wait(10) == FakeDevice.value(srv.schedule_for("a", second.adj, second.settings))
+
+
+def test_max_batch_limit():
+    log = {"in_flight": 0, "peak": 0, "sizes": []}
+    for bad in (0, 257):
+        try:
+            _server(log, max_batch=bad)
+            assert False, bad
+        except ValueError:
+            pass
