@@ -30,6 +30,19 @@ class GPError(RuntimeError):
         self.code = code
 
 
+class GPTimeout(GPError):
+    """GP_ETIME: gp_tuning.time_limit_s ran out between two launches.
+    .partial is the CountResult of the task ranges counted before it (its
+    count is None; stats.tasks of stats.tasks_total were counted)."""
+
+    def __init__(self, code, msg, partial=None):
+        super().__init__(code, msg)
+        self.partial = partial
+
+
+GP_ETIME = -7
+
+
 class SchedFlat(ctypes.Structure):
     _fields_ = [
         ("size", ctypes.c_int32),
@@ -419,11 +432,16 @@ class DeviceGraph:
                                      ctypes.byref(self._h)))
 
     def count(self, sched, tuning=None):
+        """gp_count.  Raises GPTimeout (with the partial sum) when the
+        tuning's time_limit_s runs out."""
         raw = ctypes.c_uint64()
         cnt = ctypes.c_int64()
         st = Stats()
-        _check(lib().gp_count(self._h, ctypes.byref(sched), ctypes.byref(tuning) if tuning else None,
-                              ctypes.byref(raw), ctypes.byref(cnt), ctypes.byref(st)))
+        rc = lib().gp_count(self._h, ctypes.byref(sched), ctypes.byref(tuning) if tuning else None,
+                            ctypes.byref(raw), ctypes.byref(cnt), ctypes.byref(st))
+        if rc == GP_ETIME:
+            raise GPTimeout(rc, lib().gp_last_error().decode(errors="replace"), CountResult(None, raw.value, st))
+        _check(rc)
         return CountResult(cnt.value, raw.value, st)
 
     def count_batch(self, scheds, tuning=None):

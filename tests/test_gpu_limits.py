@@ -1,0 +1,205 @@
+"""Device paths at the API's limits, against the oracle (bit for bit):
+7- and 8-vertex patterns (IEP width up to 6, loop depths up to 7), the
+deferred-task list overflowing into in-place runs, counts split into bounded
+launches, the host-side time limit, and the C++ drop-in itself on the device."""
+import ctypes
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import graphpi_amd as gp
+import oracle_binding as ob
+from conftest import load_json
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _random_graph(n, p, seed, hubs=0):
+    rng = np.random.default_rng(seed)
+    a = rng.random((n, n)) < p
+    for h in range(hubs):
+        a[h, :] = rng.random(n) < 0.7
+    a = np.triu(a, 1)
+    return gp.HostGraph.from_edges(np.argwhere(a))
+
+
+def _star(k):
+    return gp.adj_from_edges(k, [(0, i) for i in range(1, k)])
+
+
+def _cycle(k):
+    return gp.adj_from_edges(k, [(i, (i + 1) % k) for i in range(k)])
+
+
+BIG = {
+    "clique_7_minus": gp.PATTERNS["clique_7_minus"],
+    "star7": _star(7),                       # IEP over 6 trailing vertices: Bell(6) = 203 groups
+    "cycle7": _cycle(7),
+    "cycle8": _cycle(8),                     # leaf at depth 7
+    "wheel7": gp.adj_from_edges(7, [(0, i) for i in range(1, 7)] + [(i, i % 6 + 1) for i in range(1, 7)]),
+    "two_squares8": gp.adj_from_edges(8, [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (1, 4), (4, 5), (5, 6), (6, 7),
+                                           (7, 4)]),
+    "path8": gp.adj_from_edges(8, [(i, i + 1) for i in range(7)]),
+    "clique8_minus": gp.adj_from_edges(8, [(j, i) for i in range(8) for j in range(i) if (i + j) % 5]),
+}
+
+CASES = [
+    ("dense40", ["clique_7_minus", "star7", "cycle7", "wheel7", "two_squares8", "clique8_minus"],
+     [(1, 1, 1), (1, 1, 0)]),
+    ("dense40", ["cycle8", "path8"], [(1, 1, 1)]),
+    ("rand60", ["star7", "cycle7", "two_squares8", "wheel7", "clique8_minus"], [(1, 1, 1), (1, 1, 0)]),
+    ("rmat10", ["star7", "clique_7_minus", "wheel7", "clique8_minus"], [(1, 1, 1)]),
+]
+
+
+def _graph(name):
+    if name == "dense40":
+        return _random_graph(40, 0.55, 23)
+    if name == "rand60":
+        return _random_graph(60, 0.3, 21, hubs=2)
+    return gp.HostGraph.rmat(10, 8, seed=9)
+
+
+@pytest.mark.parametrize("gname,names,settings", CASES)
+def test_seven_and_eight_vertex_patterns(gname, names, settings):
+    host = _graph(gname)
+    dev = gp.DeviceGraph(host)
+    tri = ob.triangle_counting(host)
+    scheds = []
+    for name in names:
+        for st in settings:
+            s, valid = gp.build_schedule(BIG[name], *st, host.v_cnt, host.e_cnt, tri)
+            assert valid
+            want = ob.pattern_matching(host, s, threads=8)
+            got = dev.count(s)
+            print(f"{gname} {name} {st}: {got.count} ({got.stats.kernel_ms:.2f} ms)", flush=True)
+            assert (got.raw, got.count) == (want["raw"], want["count"]), (gname, name, st)
+            scheds.append((s, want))
+    # the same schedules in one batch
+    res, _ = dev.count_batch([s for s, _ in scheds])
+    for (s, want), r in zip(scheds, res):
+        assert (r.raw, r.count) == (want["raw"], want["count"])
+
+
+def test_iep_wider_than_six_is_rejected():
+    # a star with 7 leaves needs IEP over 7 trailing vertices (Bell(7) = 877
+    # groups), beyond the flattened schedule's tables
+    with pytest.raises(gp.GPError) as e:
+        gp.build_schedule(_star(8), 1, 1, True, 100, 1000, 10)
+    assert e.value.code == -3
+
+
+def test_deferred_list_overflow_runs_in_place():
+    """split_min = 1 defers every House task with more than one depth-2
+    candidate; R-MAT s17 (edge factor 16) has about 1.86M of them, more than
+    the 2^20 entries of the deferred list, so the rest run in place."""
+    host = gp.HostGraph.rmat(17, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = dev.triangle_counting()
+    s, _ = gp.build_schedule(gp.PATTERNS["house"], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+    over = gp.Tuning()
+    over.split_min = 1
+    over.slice_ms = -1
+    r = dev.count(s, over)
+    print(f"deferred {r.stats.split_tasks}, in place {r.stats.inplace_tasks}, {r.stats.kernel_ms:.1f} ms", flush=True)
+    assert r.stats.split_tasks == 1 << 20 and r.stats.inplace_tasks > 100000
+    never = gp.Tuning()
+    never.split_min = -1
+    never.slice_ms = -1
+    base = dev.count(s, never)
+    assert base.stats.split_tasks == 0
+    assert r.raw == base.raw == dev.count(s).raw
+
+
+def test_bounded_launches_equal_one_launch():
+    host = gp.HostGraph.rmat(16, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = dev.triangle_counting()
+    assert dev.count_tasks(gp.build_schedule(gp.PATTERNS["house"], 1, 1, True, host.v_cnt, host.e_cnt, tri)[0]) \
+        > 1 << 20
+    tiny = gp.Tuning()
+    tiny.slice_ms = 1
+    one = gp.Tuning()
+    one.slice_ms = -1
+    for name in ["house", "4clique", "pentagon", "rectangle"]:
+        s, _ = gp.build_schedule(gp.PATTERNS[name], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+        a, b, c = dev.count(s, tiny), dev.count(s, one), dev.count(s)
+        print(f"{name}: {a.stats.launches} launches of about 1 ms, {c.stats.launches} by default", flush=True)
+        assert a.stats.launches > 2 and b.stats.launches == 1
+        assert a.stats.tasks_total == b.stats.tasks_total == dev.count_tasks(s)
+        assert (a.raw, a.count) == (b.raw, b.count) == (c.raw, c.count), name
+
+
+def test_time_limit_stops_between_launches():
+    host = gp.HostGraph.rmat(16, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = dev.triangle_counting()
+    s, _ = gp.build_schedule(gp.PATTERNS["house"], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+    t = gp.Tuning()
+    t.slice_ms = 1
+    t.time_limit_s = 1e-6  # passed by the time the first launch returns
+    with pytest.raises(gp.GPTimeout) as e:
+        dev.count(s, t)
+    part = e.value.partial
+    assert e.value.code == gp.GP_ETIME and part.count is None
+    assert part.stats.finished == 0 and part.stats.launches == 1
+    assert 0 < part.stats.tasks < part.stats.tasks_total
+    # the partial sum is the count of the range that ran
+    r = gp.Tuning()
+    r.task_end = int(part.stats.tasks)
+    r.slice_ms = -1
+    assert dev.count(s, r).raw == part.raw
+    # a limit that is not reached changes nothing
+    t.time_limit_s = 3600.0
+    assert dev.count(s, t).raw == dev.count(s).raw
+
+
+def test_cpp_drop_in_counts_on_the_device(tmp_path, wiki_vote_path):
+    """A driver shaped like the reference's tianhe/baseline_test.cpp, compiled
+    against include/ and linked with the library: DataLoader, Pattern,
+    Schedule, then g->pattern_matching(s, 24), which runs on the device."""
+    src = tmp_path / "baseline.cpp"
+    src.write_text(r'''
+#include "common.h"
+#include "dataloader.h"
+#include "graph.h"
+#include "pattern.h"
+#include "schedule.h"
+#include <cstdio>
+int main(int argc, char** argv) {
+    Graph* g;
+    DataLoader D;
+    DataType type;
+    GetDataType(type, "Wiki-Vote");
+    if (!D.load_data(g, type, argv[1])) return 1;
+    const char* pats[2] = {"0111010011100011100001100", "0111101111011110"};  // House, 4-clique
+    const int sizes[2] = {5, 4};
+    for (int i = 0; i < 2; ++i) {
+        Pattern p(sizes[i], pats[i]);
+        bool valid;
+        Schedule s(p, valid, 1, 1, true, g->v_cnt, g->e_cnt, g->tri_cnt);
+        if (!valid) return 2;
+        double t0 = get_wall_time();
+        long long ans = g->pattern_matching(s, 24);
+        std::printf("ans %lld\ntime %.6f\nlaunches %d\n", ans, get_wall_time() - t0, g->last_stats.launches);
+    }
+    delete g;
+    return 0;
+}
+''')
+    exe = tmp_path / "baseline"
+    r = subprocess.run(["g++", "-std=c++11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        gp.LIB_PATH, "-Wl,-rpath," + os.path.dirname(gp.LIB_PATH)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe), wiki_vote_path], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    ans = [int(l.split()[1]) for l in out.stdout.splitlines() if l.startswith("ans ")]
+    gold = {r["pattern"]: r["count"] for r in load_json("reference_counts.json")
+            if r["graph"] == "wiki-vote" and (r["pm"], r["rt"], r["iep"]) == (1, 1, 1)}
+    assert ans == [gold["house"], gold["4clique"]]
