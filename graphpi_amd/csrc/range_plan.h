@@ -6,7 +6,10 @@
 // Rules: a count of at most 2^20 tasks, or one with slice_ms < 0, is one
 // launch.  Otherwise the first launch takes 1/256 of the tasks; each later
 // range is sized from the previous range's kernel time per task for about
-// slice_ms, at most twice the previous range; the rest goes into one launch
+// slice_ms, at most twice the previous range and at least 1/1024 of the
+// tasks (a launch's fixed cost must not shrink the ranges to nothing when it
+// alone exceeds slice_ms, so a count makes at most about 1,300 launches);
+// the rest goes into one launch
 // when it is at most 1.25 ranges, or, after the first launch, when it is
 // predicted to fit one slice even at four times the first range's rate.  After each launch but the last, the wall time since the start is
 // compared with the time limit (the reference's Graph::max_running_time,
@@ -56,8 +59,9 @@ struct GpRangePlan {
         if (launches == 1) first_rate = rate;
         const uint64_t left = end - cur;
         const double fit = slice_ms / rate;
-        size = std::max<uint64_t>(1, std::min<uint64_t>(2 * std::max<uint64_t>(did, 1),
-                                                        fit >= 1.8e19 ? ~0ull : (uint64_t)fit));
+        const uint64_t floor_size = std::max<uint64_t>(1, (end - begin) / 1024);
+        size = std::max<uint64_t>(floor_size, std::min<uint64_t>(2 * std::max<uint64_t>(did, 1),
+                                                                 fit >= 1.8e19 ? ~0ull : (uint64_t)fit));
         if (left <= size + size / 4 || (launches == 1 && 4.0 * first_rate * (double)left < slice_ms))
             size = left;
     }

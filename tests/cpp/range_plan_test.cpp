@@ -89,6 +89,24 @@ int main() {
         CHECK(mx < 10.0 * 1000.0 && p.launches >= 6);
         std::printf("rising: %d launches, longest %.0f ms\n", p.launches, mx);
     }
+    // a fixed cost per launch above the slice target: ranges keep a floor of
+    // 1/1024 of the tasks instead of shrinking to single tasks
+    ms.clear();
+    {
+        double now = 0;
+        GpRangePlan p(0, 2000000, 1, 0, now);
+        uint64_t x = 0, y = 0, covered = 0;
+        while (p.next(&x, &y)) {
+            const double k = 2.0 + 1e-7 * (double)(y - x); // 2 ms whatever the range
+            covered += y - x;
+            now += k / 1000.0;
+            p.finished(y, k, now);
+            CHECK(p.launches < 5000);
+            if (p.launches >= 5000) break;
+        }
+        CHECK(covered == 2000000 && p.launches <= 1300);
+        std::printf("fixed cost: %d launches\n", p.launches);
+    }
     // time limit: stops between launches, with part of the tasks counted
     ms.clear();
     {

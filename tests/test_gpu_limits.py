@@ -122,14 +122,16 @@ def test_bounded_launches_equal_one_launch():
     tri = dev.triangle_counting()
     assert dev.count_tasks(gp.build_schedule(gp.PATTERNS["house"], 1, 1, True, host.v_cnt, host.e_cnt, tri)[0]) \
         > 1 << 20
-    tiny = gp.Tuning()
-    tiny.slice_ms = 1
     one = gp.Tuning()
     one.slice_ms = -1
-    for name in ["house", "4clique", "pentagon", "rectangle"]:
+    for name, ms in [("house", 1), ("4clique", 5), ("pentagon", 1), ("rectangle", 5)]:
+        tiny = gp.Tuning()
+        tiny.slice_ms = ms  # launches of about ms each (at least 1/1024 of the tasks)
         s, _ = gp.build_schedule(gp.PATTERNS[name], 1, 1, True, host.v_cnt, host.e_cnt, tri)
         a, b, c = dev.count(s, tiny), dev.count(s, one), dev.count(s)
-        print(f"{name}: {a.stats.launches} launches of about 1 ms, {c.stats.launches} by default", flush=True)
+        assert a.stats.launches <= 1300
+        print(f"{name}: {a.stats.launches} launches at slice_ms {ms} ({a.stats.kernel_ms:.1f} ms), "
+              f"{c.stats.launches} by default", flush=True)
         assert a.stats.launches > 2 and b.stats.launches == 1
         assert a.stats.tasks_total == b.stats.tasks_total == dev.count_tasks(s)
         assert (a.raw, a.count) == (b.raw, b.count) == (c.raw, c.count), name

@@ -73,10 +73,16 @@ def test_rmat22_4clique_whole_count(rmat22):
           f"(one launch {single.stats.kernel_ms:.1f} ms)", flush=True)
     assert got.stats.launches > 1 and single.stats.launches == 1
     assert (got.raw, got.count) == (single.raw, single.count)
+    # the oracle's whole count as 16 interleaved root slices (roots i, i + 16,
+    # ...), whose raw sums add up; each slice reports when it is done
     t0 = time.time()
-    want = ob.pattern_matching(host, s, threads=_threads())
-    print(f"4-clique oracle {want['count']} in {time.time() - t0:.1f} s on {_threads()} threads", flush=True)
-    assert (got.raw, got.count) == (want["raw"], want["count"]) == (12898716441, 12898716441)
+    raw = 0
+    for i in range(16):
+        part = ob.pattern_matching(host, s, threads=_threads(), root_begin=i, root_stride=16)
+        raw = (raw + part["raw"]) % (1 << 64)
+        print(f"  oracle 4-clique slice {i + 1}/16: {time.time() - t0:.1f} s", flush=True)
+    print(f"4-clique oracle raw {raw} in {time.time() - t0:.1f} s on {_threads()} threads", flush=True)
+    assert got.raw == raw and got.count == 12898716441 == raw // (s.redundancy or 1)
 
 
 HOUSE_ROOTS = [(40, 41), (4000, 4001), (40000, 40016), (400000, 400256), (1000000, 1016384)]
