@@ -119,6 +119,7 @@ class Tuning(ctypes.Structure):
         ("no_pair_v1", ctypes.c_int32),
         ("bitmap_ratio", ctypes.c_int32),
         ("slice_ms", ctypes.c_int32),
+        ("no_special", ctypes.c_int32),
         ("time_limit_s", ctypes.c_double),
     ]
 
@@ -143,7 +144,8 @@ class Stats(ctypes.Structure):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("leaf", "iep_k", "root_tasks", "batched", "pair", "pair_grouped_v1",
-                                               "shared_search", "blocks", "kind1_blocks", "scratch_slots")]
+                                               "shared_search", "blocks", "kind1_blocks", "scratch_slots",
+                                               "shape")]
 
 
 class _Csr(ctypes.Structure):

@@ -35,6 +35,7 @@
 
 #include "../gp_internal.h"
 #include "../range_plan.h"
+#include "house.h"
 #include "kplan.h"
 #include "schedule.h"
 
@@ -1688,7 +1689,10 @@ struct gp_graph {
     uint32_t* d_vertex = nullptr;
     int32_t* d_edge = nullptr;
     int32_t* d_src = nullptr;
+    int32_t* d_sup = nullptr;   // per edge slot |N(src) ∩ N(dst)|, built on first use
+    double sup_build_ms = 0;    // time of that build
     std::mutex mu;
+    std::mutex sup_mu;
     std::vector<CallCtx*> idle;
 };
 
@@ -2109,11 +2113,156 @@ hipError_t graph_run(CallCtx* c, const void* kfun, const void* pfun, unsigned gr
     return hipSuccess;
 }
 
+// The edge-support array of a graph (the House kernel reads it), built on
+// the first call that needs it and kept with the graph.
+int ensure_support(gp_graph* g, CallCtx* c) {
+    std::lock_guard<std::mutex> lk(g->sup_mu);
+    if (g->d_sup || g->e_cnt == 0) return GP_OK;
+    int32_t* sup = nullptr;
+    HIP_TRY(hipMalloc((void**)&sup, sizeof(int32_t) * (size_t)g->e_cnt));
+    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, 8);
+        e = hipGetLastError();
+    }
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+    if (e == hipSuccess)
+        e = gp_support_launch(g->d_vertex, g->d_edge, g->d_src, g->e_cnt, sup, c->d_cnt, (unsigned)(g->num_cu * 16),
+                              c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
+    if (e != hipSuccess) {
+        (void)hipFree(sup);
+        return gp_fail((int)e, std::string("edge supports: ") + hipGetErrorString(e));
+    }
+    g->d_sup = sup;
+    g->sup_build_ms = ms;
+    return GP_OK;
+}
+
+// House under the GraphPi schedule's order, with the specialised two-hop
+// kernel (house.hip): one launch over a task range of edge slots.
+int run_house(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, bool all_v1, uint64_t* raw,
+              int64_t* count, gp_stats* stats) {
+    HIP_TRY(hipSetDevice(g->device));
+    const uint64_t all_tasks = (uint64_t)g->e_cnt;
+    uint64_t t_begin = 0, t_end = all_tasks;
+    if (tuning && (tuning->task_begin != 0 || tuning->task_end > 0)) {
+        if (tuning->task_begin < 0 || (tuning->task_end > 0 && tuning->task_end < tuning->task_begin))
+            return gp_fail(GP_EINVAL, "bad task range");
+        t_begin = std::min<uint64_t>((uint64_t)tuning->task_begin, all_tasks);
+        if (tuning->task_end > 0) t_end = std::min<uint64_t>((uint64_t)tuning->task_end, all_tasks);
+    }
+    const int wpc = tuning && tuning->waves_per_cu > 0 ? tuning->waves_per_cu : 24;
+    int64_t waves = (int64_t)g->num_cu * wpc;
+    // one dense count array per wave: fewer waves when they would not fit
+    const int64_t stride = ((int64_t)std::max(g->v_cnt, 1) + 63) / 64 * 64;
+    const int64_t fit = (int64_t)(GP_DENSE_MAX_BYTES / ((uint64_t)stride * sizeof(int32_t)));
+    waves = std::max<int64_t>(1, std::min<int64_t>(waves, fit));
+    if (t_end > t_begin) waves = std::min<int64_t>(waves, (int64_t)(t_end - t_begin));
+    const size_t dense_bytes = (size_t)waves * (size_t)stride * sizeof(int32_t);
+    CallCtx* c = nullptr;
+    int rc = acquire_ctx(g, 64, &c);
+    if (rc != GP_OK) return rc;
+    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess && dense_bytes > c->dense_bytes) {
+        e = ensure((void**)&c->d_dense, &c->dense_bytes, dense_bytes);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(zero_i32_kernel, dim3(4096), dim3(256), 0, c->stream, c->d_dense,
+                               (uint64_t)(c->dense_bytes / sizeof(int32_t)));
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess && c->clocks_cap < (size_t)waves) {
+        size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
+        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves);
+        c->clocks_cap = e == hipSuccess ? (size_t)waves : 0;
+    }
+    if (e != hipSuccess) {
+        destroy_ctx(c);
+        return gp_fail(GP_ENOMEM, std::string("call buffers: ") + hipGetErrorString(e));
+    }
+    const bool built_now = g->d_sup == nullptr;
+    rc = ensure_support(g, c);
+    if (rc != GP_OK) {
+        destroy_ctx(c);
+        return rc;
+    }
+    GpHouseArgs a;
+    a.vertex = g->d_vertex;
+    a.edge = g->d_edge;
+    a.sup = g->d_sup;
+    a.v_cnt = g->v_cnt;
+    a.slot_begin = t_begin;
+    a.slot_end = t_end;
+    a.all_v1 = all_v1 ? 1 : 0;
+    a.counter = c->d_cnt + 1;
+    a.dense = c->d_dense;
+    a.dense_stride = stride;
+    a.out = c->d_cnt + 3;
+    a.clocks = c->d_clocks;
+    unsigned long long host_out[8] = {0};
+    std::vector<unsigned long long> clocks(2 * (size_t)waves, 0);
+    float ms = 0;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, 8);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+    if (e == hipSuccess && t_end > t_begin) e = gp_house_launch(a, (unsigned)waves, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && t_end > t_begin)
+        e = hipMemcpyAsync(clocks.data(), c->d_clocks, sizeof(unsigned long long) * 2 * (size_t)waves,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->e0, c->e1);
+    if (e != hipSuccess) {
+        destroy_ctx(c);
+        return gp_fail((int)e, std::string("House kernel: ") + hipGetErrorString(e));
+    }
+    release_ctx(g, c);
+    const uint64_t sum = host_out[3];
+    if (raw) *raw = sum;
+    if (count) *count = (int64_t)sum / (sched->redundancy ? sched->redundancy : 1);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->kernel_ms = ms + (built_now ? g->sup_build_ms : 0.0);
+        stats->tasks = host_out[5];
+        stats->waves = (int32_t)waves;
+        stats->kernel_variant = 2;
+        stats->launches = 1;
+        stats->finished = 1;
+        stats->tasks_total = t_end > t_begin ? t_end - t_begin : 0;
+        if (t_end > t_begin) {
+            int rate_khz = 0;
+            if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
+                rate_khz <= 0)
+                rate_khz = 100000;
+            unsigned long long t0 = ~0ull;
+            for (int64_t w = 0; w < waves; ++w) t0 = std::min(t0, clocks[2 * w]);
+            std::vector<double> fin((size_t)waves);
+            for (int64_t w = 0; w < waves; ++w) fin[w] = (double)(clocks[2 * w + 1] - t0) / rate_khz;
+            std::sort(fin.begin(), fin.end());
+            stats->finish_p50_ms = fin[fin.size() / 2];
+            stats->finish_p90_ms = fin[fin.size() * 9 / 10];
+            stats->finish_max_ms = fin.back();
+        }
+    }
+    return GP_OK;
+}
+
 // Count nplans schedules over one graph in one persistent launch (plus the
 // launch of deferred parts).  Results are per schedule; a task range in the
 // tuning is only accepted for a single schedule.
 int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, uint64_t* raw,
               int64_t* count, gp_stats* stats) {
+    // schedule-specialised kernels for single counts of shapes they cover
+    if (nplans == 1 && !(tuning && (tuning->collect_balg || tuning->no_special))) {
+        const int shape = gp_sched_shape(&scheds[0]);
+        if (shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL)
+            return run_house(g, scheds, tuning, shape == GP_SHAPE_HOUSE_ALL, raw, count, stats);
+    }
     Launch ln;
     {
         int rc = prepare(g, scheds, nplans, tuning, &ln);
@@ -2380,13 +2529,14 @@ extern "C" int gp_graph_free(gp_graph* g) {
     if (g->d_vertex) (void)hipFree(g->d_vertex);
     if (g->d_edge) (void)hipFree(g->d_edge);
     if (g->d_src) (void)hipFree(g->d_src);
+    if (g->d_sup) (void)hipFree(g->d_sup);
     delete g;
     return GP_OK;
 }
 
 extern "C" int gp_graph_bytes(const gp_graph* g, uint64_t* bytes) {
     if (!g || !bytes) return gp_fail(GP_EINVAL, "null argument");
-    *bytes = sizeof(uint32_t) * ((uint64_t)g->v_cnt + 1) + 2 * sizeof(int32_t) * (uint64_t)g->e_cnt;
+    *bytes = sizeof(uint32_t) * ((uint64_t)g->v_cnt + 1) + (g->d_sup ? 3 : 2) * sizeof(int32_t) * (uint64_t)g->e_cnt;
     return GP_OK;
 }
 

@@ -1,0 +1,25 @@
+// Launch interface of the House-specialised kernel (house.hip) and of the
+// edge-support kernel it reads.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+struct GpHouseArgs {
+    const uint32_t* vertex;
+    const int32_t* edge;
+    const int32_t* sup;        // per edge slot: |N(src) ∩ N(dst)|
+    int32_t v_cnt;
+    uint64_t slot_begin;       // task range: edge slots [slot_begin, slot_end)
+    uint64_t slot_end;
+    int32_t all_v1;            // 1: every v1 in N(v0) (no restriction v1 < v0)
+    unsigned long long* counter; // row fetch counter, zero at launch
+    int32_t* dense;            // per-wave count arrays over all vertex ids, zero at launch (left zero)
+    int64_t dense_stride;
+    unsigned long long* out;   // [0] raw sum, [2] tasks taken
+    unsigned long long* clocks; // per wave: start, finish
+};
+
+hipError_t gp_house_launch(const GpHouseArgs& args, unsigned waves, hipStream_t stream);
+hipError_t gp_support_launch(const uint32_t* vertex, const int32_t* edge, const int32_t* src, uint64_t e_cnt,
+                             int32_t* sup, unsigned long long* counter, unsigned waves, hipStream_t stream);

@@ -80,5 +80,12 @@ struct KPlan {
     int32_t split_min;
 };
 
+// Loop-nest shapes with a schedule-specialised kernel.
+#define GP_SHAPE_GENERIC 0
+#define GP_SHAPE_HOUSE 1     // House in the GraphPi order: v1 < v0 in N(v0), v2 in N(v0), IEP over
+                             // v3 in N(v0) ∩ N(v1) and v4 in N(v1) ∩ N(v2)
+#define GP_SHAPE_HOUSE_ALL 2 // the same loop nest without the restriction v1 < v0
+int gp_sched_shape(const gp_sched_flat* s);
+
 // Returns GP_OK or an error code (with gp_fail message) for unsupported input.
 int gp_make_kplan(const gp_sched_flat* s, bool faithful, bool allow_batch, bool allow_pair, KPlan* out);

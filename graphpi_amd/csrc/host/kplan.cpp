@@ -320,5 +320,69 @@ extern "C" int gp_kernel_plan(const gp_sched_flat* sched, gp_plan_info* out) {
     out->blocks = kp.batched ? kp.nblocks : 0;
     out->kind1_blocks = kp.batched ? kp.nk1 : 0;
     out->scratch_slots = kp.nslots;
+    out->shape = gp_sched_shape(sched);
     return GP_OK;
+}
+
+// The loop nest a flattened schedule describes, compared with the shapes
+// that have a specialised kernel.  Each prefix's set is the intersection of
+// N(v_j) over the depths j on its father chain (a prefix whose last element
+// is depth d adds N(v_d)); the loop set of depth d must be the intersection
+// over d's earlier neighbours in the schedule-ordered adjacency, as the
+// planner builds it (src/schedule.cpp:329-362).
+int gp_sched_shape(const gp_sched_flat* s) {
+    const int n = s->size;
+    if (n != 5 || s->iep_num != 2 || s->iep_group_num != 2) return GP_SHAPE_GENERIC;
+    static const int house[5][5] = {{0, 1, 1, 1, 0}, {1, 0, 0, 1, 1}, {1, 0, 0, 0, 1}, {1, 1, 0, 0, 0}, {0, 1, 1, 0, 0}};
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if ((s->adj[i * n + j] != 0) != (house[i][j] != 0)) return GP_SHAPE_GENERIC;
+    // prefix p -> depth mask of the neighbourhoods it intersects
+    const int P = s->total_prefix_num;
+    if (P < 1 || P > GP_MAX_PREFIX) return GP_SHAPE_GENERIC;
+    int depth_of[GP_MAX_PREFIX];
+    for (int p = 0; p < P; ++p) depth_of[p] = -1;
+    for (int d = 0; d < n; ++d)
+        for (int p = s->last[d], guard = 0; p != -1 && guard <= P; p = s->next[p], ++guard) {
+            if (p < 0 || p >= P) return GP_SHAPE_GENERIC;
+            depth_of[p] = d;
+        }
+    uint32_t mask[GP_MAX_PREFIX];
+    for (int p = 0; p < P; ++p) {
+        uint32_t m = 0;
+        for (int q = p, guard = 0; q != -1 && guard <= P; q = s->father[q], ++guard) {
+            if (q < 0 || q >= P || depth_of[q] < 0) return GP_SHAPE_GENERIC;
+            m |= 1u << depth_of[q];
+        }
+        mask[p] = m;
+    }
+    for (int d = 1; d < n; ++d) {
+        const int q = s->loop_set[d];
+        if (q < 0 || q >= P) return GP_SHAPE_GENERIC;
+        uint32_t want = 0;
+        for (int j = 0; j < d; ++j)
+            if (house[d][j]) want |= 1u << j;
+        if (mask[q] != want) return GP_SHAPE_GENERIC;
+    }
+    // restrictions: none, or only v1 < v0 (checked at depth 1)
+    if (s->total_restrict_num != 0 && s->total_restrict_num != 1) return GP_SHAPE_GENERIC;
+    for (int d = 0; d < n; ++d) {
+        const int r = s->restrict_last[d];
+        if (d == 1 && s->total_restrict_num == 1) {
+            if (r < 0 || r >= GP_MAX_RESTRICT || s->restrict_index[r] != 0 || s->restrict_next[r] != -1)
+                return GP_SHAPE_GENERIC;
+        } else if (r != -1) {
+            return GP_SHAPE_GENERIC;
+        }
+    }
+    // IEP over v3, v4: |A||B| - |A ∩ B| (two singleton blocks with weight +1,
+    // one block of both with weight -1), in either group order
+    int singles = -1, joint = -1;
+    for (int g = 0; g < 2; ++g) {
+        const int b0 = s->iep_block[g][0], b1 = s->iep_block[g][1];
+        if (b0 != b1 && s->iep_val[g] == 1) singles = g;
+        if (b0 == b1 && s->iep_val[g] == -1) joint = g;
+    }
+    if (singles < 0 || joint < 0) return GP_SHAPE_GENERIC;
+    return s->total_restrict_num == 1 ? GP_SHAPE_HOUSE : GP_SHAPE_HOUSE_ALL;
 }

@@ -128,6 +128,9 @@ typedef struct gp_tuning {
                                many milliseconds of kernel time (a first launch takes 1/256 of
                                the tasks when there are more than 2^20); 0 = default (2000),
                                < 0 = one launch */
+    int32_t no_special;     /* 1 = never use a schedule-specialised kernel (House under the
+                               GraphPi schedule is counted per root with two-hop counts and
+                               edge supports by default); counts are identical either way */
     double time_limit_s;    /* gp_count: when > 0, stop between launches once this much wall
                                time has passed and return GP_ETIME (the reference's
                                Graph::max_running_time, include/graph.h:12); every launch
@@ -220,6 +223,9 @@ typedef struct gp_plan_info {
     int32_t blocks;          /* leaf blocks of the batched level (1 for a plain leaf) */
     int32_t kind1_blocks;    /* blocks intersected with each candidate's list */
     int32_t scratch_slots;   /* prefix sets materialised per wave */
+    int32_t shape;           /* schedule-specialised kernel a single count takes: 0 = none (the
+                                generic kernel), 1 = House with v1 < v0 (two-hop counts per root),
+                                2 = the same House loop nest without that restriction */
 } gp_plan_info;
 GP_API int gp_kernel_plan(const gp_sched_flat* sched, gp_plan_info* out);
 

@@ -531,3 +531,38 @@ def test_sampled_index_matches():
         small.lds_stage = stage
         got = dev.count(s, small)
         assert (got.raw, got.count) == (want["raw"], want["count"]), stage
+
+@pytest.mark.parametrize("graph", ["rmat13", "hubs", "chunglu", "wiki"])
+def test_house_twohop_kernel_matches(graph, rmat13_path, wiki_vote_path):
+    """House under the GraphPi schedule runs the specialised two-hop kernel
+    (per-root counts |N(x) ∩ N(v0)| and edge supports); it must equal the
+    generic kernel (no_special), the oracle, and add up over task ranges;
+    without restrictions (rt = 0) the unrestricted variant runs."""
+    if graph == "rmat13":
+        host = gp.HostGraph.load_text(rmat13_path)
+    elif graph == "hubs":
+        host = _hub_graph()
+    elif graph == "chunglu":
+        host = gp.HostGraph.chung_lu(6000, 40000, 2.1, 800.0, 3)
+    else:
+        host = gp.HostGraph.load_text(wiki_vote_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    generic = gp.Tuning()
+    generic.no_special = 1
+    for rt, shape in [(1, 1), (0, 2)]:
+        s = _sched(host, gp.PATTERNS["house"], 1, rt, 1, tri)
+        assert gp.kernel_plan(s)["shape"] == shape
+        got = dev.count(s)
+        assert got.stats.kernel_variant == 2
+        want = ob.pattern_matching(host, s, threads=8)
+        gen = dev.count(s, generic)
+        print(f"{graph} House rt={rt}: {got.count} two-hop {got.stats.kernel_ms:.2f} ms, generic "
+              f"{gen.stats.kernel_ms:.2f} ms", flush=True)
+        assert (got.raw, got.count) == (want["raw"], want["count"]) == (gen.raw, gen.count), (graph, rt)
+        for slices in (3, 17):
+            assert dev.count_sliced(s, slices).raw == want["raw"], (graph, rt, slices)
+    if graph == "wiki":
+        gold = [r["count"] for r in load_json("reference_counts.json")
+                if r["graph"] == "wiki-vote" and r["pattern"] == "house" and (r["pm"], r["rt"], r["iep"]) == (1, 1, 1)]
+        assert dev.pattern_matching(_sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)) == gold[0]

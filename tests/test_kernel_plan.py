@@ -53,3 +53,29 @@ def test_every_motif_has_a_plan():
             assert not info["shared_search"] or info["kind1_blocks"] >= 2
             n += 1
     assert n == 2 + 6 + 21 + 112
+
+
+def test_specialised_shapes():
+    # House under the GraphPi schedule takes the two-hop kernel (shape 1);
+    # without restrictions the same loop nest is shape 2; other orders and
+    # every other pattern stay on the generic kernel (shape 0)
+    def shape(name, pm=1, rt=1, iep=True):
+        s, valid = gp.build_schedule(gp.PATTERNS[name], pm, rt, iep, *STATS)
+        assert valid
+        return gp.kernel_plan(s)["shape"]
+    assert shape("house") == 1
+    assert shape("house", 0, 1) == 1
+    assert shape("house", 1, 0) == 2
+    assert shape("house", 1, 1, False) == 0   # no IEP: a plain leaf at depth 4
+    assert shape("house", 2, 1) == 0          # the GraphZero model picks another vertex order
+    for name in sorted(EXPECTED):
+        if name != "house":
+            assert shape(name) == 0, name
+    # a tampered restriction (v2 < v0 added at depth 2) is not the House shape
+    s, _ = gp.build_schedule(gp.PATTERNS["house"], 1, 1, True, *STATS)
+    r = s.total_restrict_num
+    s.restrict_index[r] = 0
+    s.restrict_next[r] = -1
+    s.restrict_last[2] = r
+    s.total_restrict_num = r + 1
+    assert gp.kernel_plan(s)["shape"] == 0
