@@ -50,6 +50,9 @@ typedef struct {
     int32_t emb_n;
     int32_t* tmp; /* IEP scratch, max_degree entries */
     int32_t v1_begin, v1_end; /* depth-1 filter (test slicing; the reference has none) */
+    double deadline;          /* timed sampling: abandon the root once omp_get_wtime() passes it (0: never) */
+    int abandoned;
+    uint32_t tick;
     uint64_t ans, balg, expanded;
 } ctx_t;
 
@@ -183,6 +186,8 @@ static void level(ctx_t* c, int depth) {
 
     int32_t mv = bound_at(c, depth);
     for (int32_t i = 0; i < loop_size; ++i) {
+        if (c->deadline > 0 && (++c->tick & 1023) == 0 && omp_get_wtime() > c->deadline) c->abandoned = 1;
+        if (c->abandoned) return;
         int32_t v = loop[i];
         if (mv <= v) break;
         if (depth == 1 && (v < c->v1_begin || v >= c->v1_end)) continue;
@@ -279,6 +284,72 @@ int64_t gpo_pattern_matching_v1(const int32_t* edge, const uint32_t* vertex, int
     if (expanded) *expanded = te;
     int64_t red = s->redundancy ? s->redundancy : 1;
     return (int64_t)total / red;
+}
+
+int64_t gpo_pattern_matching_timed(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
+                                   const gp_sched_flat* s, int threads, const int32_t* roots,
+                                   int64_t n_roots, double time_limit_s, uint64_t* raw_sum,
+                                   int64_t* roots_done, double* busy_s) {
+    int32_t d1 = 0, d2 = 0;
+    for (int32_t v = 0; v < v_cnt; ++v) {
+        int32_t d = (int32_t)(vertex[v + 1] - vertex[v]);
+        if (d > d1) {
+            d2 = d1;
+            d1 = d;
+        } else if (d > d2)
+            d2 = d;
+    }
+    if (d2 < 1) d2 = 1;
+    if (d1 < 1) d1 = 1;
+    if (threads < 1) threads = 1;
+    const double deadline = omp_get_wtime() + time_limit_s;
+    uint64_t total = 0;
+    int64_t done = 0;
+    double busy = 0;
+#pragma omp parallel num_threads(threads) reduction(+ : total, done, busy)
+    {
+        ctx_t c;
+        memset(&c, 0, sizeof(c));
+        c.edge = edge;
+        c.vertex = vertex;
+        c.v_cnt = v_cnt;
+        c.s = s;
+        c.v1_begin = 0;
+        c.v1_end = v_cnt;
+        c.deadline = deadline;
+        for (int p = 0; p < s->total_prefix_num; ++p)
+            if (s->father[p] != -1) c.sets[p].buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)d2);
+        c.tmp = (int32_t*)malloc(sizeof(int32_t) * (size_t)d1);
+#pragma omp for schedule(dynamic) nowait
+        for (int64_t t = 0; t < n_roots; ++t) {
+            if (omp_get_wtime() > deadline) continue;
+            const double t0 = omp_get_wtime();
+            const uint64_t before = c.ans;
+            int32_t v = roots[t];
+            uint32_t l = vertex[v], r = vertex[v + 1];
+            for (int p = s->last[0]; p != -1; p = s->next[p])
+                build_prefix(&c, p, edge + l, (int32_t)(r - l));
+            c.emb[0] = v;
+            c.emb_n = 1;
+            c.abandoned = 0;
+            level(&c, 1);
+            c.emb_n = 0;
+            if (c.abandoned) {
+                c.ans = before; /* an unfinished root counts for nothing */
+            } else {
+                total += c.ans - before;
+                ++done;
+                busy += omp_get_wtime() - t0;
+            }
+            c.ans = before;
+        }
+        for (int p = 0; p < s->total_prefix_num; ++p) free(c.sets[p].buf);
+        free(c.tmp);
+    }
+    if (raw_sum) *raw_sum = total;
+    if (roots_done) *roots_done = done;
+    if (busy_s) *busy_s = busy;
+    return done;
 }
 
 int64_t gpo_triangle_counting(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,

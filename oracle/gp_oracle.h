@@ -37,6 +37,16 @@ int64_t gpo_pattern_matching_v1(const int32_t* edge, const uint32_t* vertex, int
                                 int64_t v1_end, uint64_t* raw_sum, uint64_t* balg,
                                 uint64_t* expanded);
 
+/* Timed sampling for a CPU baseline: the loop nest over the given roots (a
+ * sample of root ids), OpenMP dynamic chunk 1 as above, until time_limit_s
+ * has passed; a root still running then is abandoned and counts for
+ * nothing.  raw_sum: the sum over the roots that finished; roots_done: how
+ * many; busy_s: thread-seconds spent on them.  Returns roots_done. */
+int64_t gpo_pattern_matching_timed(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
+                                   const gp_sched_flat* s, int threads, const int32_t* roots,
+                                   int64_t n_roots, double time_limit_s, uint64_t* raw_sum,
+                                   int64_t* roots_done, double* busy_s);
+
 /* Graph::triangle_counting_mt restated (src/graph.cpp:112-139, :66-95). */
 int64_t gpo_triangle_counting(const int32_t* edge, const uint32_t* vertex, int32_t v_cnt,
                               int threads);

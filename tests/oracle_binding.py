@@ -34,6 +34,11 @@ def olib():
                                               ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                               ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_uint64)]
+        l.gpo_pattern_matching_timed.restype = ctypes.c_int64
+        l.gpo_pattern_matching_timed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                 ctypes.POINTER(gp.SchedFlat), ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64),
+                                                 ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]
         l.gpo_triangle_counting.restype = ctypes.c_int64
         l.gpo_triangle_counting.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int]
         l.gpo_brute_force.restype = ctypes.c_int64
@@ -73,3 +78,18 @@ def brute_force(g, adj):
     v, e = _csr(g)
     a = np.ascontiguousarray(np.asarray(adj, dtype=np.int32))
     return olib().gpo_brute_force(e.ctypes.data, v.ctypes.data, g.v_cnt, a.shape[0], a.ctypes.data)
+
+
+def pattern_matching_timed(g, sched, roots, time_limit_s, threads=8):
+    """The loop nest over the root ids in `roots` until time_limit_s passes
+    (roots still running then are abandoned).  Returns dict(raw, roots_done,
+    busy_s) over the roots that finished."""
+    v, e = _csr(g)
+    r = np.ascontiguousarray(np.asarray(roots, dtype=np.int32))
+    raw = ctypes.c_uint64()
+    done = ctypes.c_int64()
+    busy = ctypes.c_double()
+    olib().gpo_pattern_matching_timed(e.ctypes.data, v.ctypes.data, g.v_cnt, ctypes.byref(sched), threads,
+                                      r.ctypes.data, r.size, time_limit_s, ctypes.byref(raw), ctypes.byref(done),
+                                      ctypes.byref(busy))
+    return {"raw": raw.value, "roots_done": done.value, "busy_s": busy.value}
