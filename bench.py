@@ -344,7 +344,8 @@ def main():
 
     def count_pattern(name, target_ms=5000.0, tuning=None, quiet=False):
         """One full count of `name` as consecutive task ranges (progress is
-        logged per range): a first range of 1/256 of the tasks; the rest in one
+        logged per range): a first range of 1/256 of the tasks (1/4 for a
+        kernel whose unit is a root); the rest in one
         launch when that is predicted to take under 10 s even at 4 times the
         first range's rate; otherwise ranges at most twice the previous one and
         sized for about target_ms at the previous range's rate (task cost
@@ -356,7 +357,11 @@ def main():
         if tuning is not None:
             ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
         tun.slice_ms = -1  # the ranges below are the launches
-        first = total if total <= (1 << 20) else max(1, total // 256)
+        # a kernel that takes whole roots (House's two-hop kernel) gets a
+        # first range of 1/4: a short first range holds only a few heavy
+        # roots and leaves most waves idle
+        div = 4 if gp.kernel_plan(s)["shape"] else 256
+        first = total if total <= (1 << 20) else max(1, total // div)
         begin, size, raw, ms, launches = 0, first, 0, 0.0, 0
         balg_sum = 0
         t0 = time.perf_counter()

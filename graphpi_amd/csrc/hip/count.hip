@@ -2561,7 +2561,10 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
     if (t.task_begin < 0 || (t.task_end > 0 && t.task_end < t.task_begin)) return gp_fail(GP_EINVAL, "bad task range");
     const uint64_t begin = std::min<uint64_t>((uint64_t)t.task_begin, all_tasks);
     const uint64_t end = t.task_end > 0 ? std::min<uint64_t>((uint64_t)t.task_end, all_tasks) : all_tasks;
-    GpRangePlan plan(begin, end, t.slice_ms, t.time_limit_s, wall_seconds());
+    // the House kernel's unit of work is a whole root, the generic kernel's an edge slot
+    const int shape = (t.collect_balg || t.no_special) ? GP_SHAPE_GENERIC : gp_sched_shape(sched);
+    const bool per_root = shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL;
+    GpRangePlan plan(begin, end, t.slice_ms, t.time_limit_s, wall_seconds(), per_root ? 4 : 256);
     if (plan.single()) return run_plans(g, sched, 1, &t, raw_sum, count, stats);
     gp_stats acc{};
     uint64_t raw = 0, b = 0, e = 0;

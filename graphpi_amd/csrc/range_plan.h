@@ -4,7 +4,8 @@
 // times.
 //
 // Rules: a count of at most 2^20 tasks, or one with slice_ms < 0, is one
-// launch.  Otherwise the first launch takes 1/256 of the tasks; each later
+// launch.  Otherwise the first launch takes 1/256 of the tasks (1/first_div);
+// each later
 // range is sized from the previous range's kernel time per task for about
 // slice_ms, at most twice the previous range and at least 1/1024 of the
 // tasks (a launch's fixed cost must not shrink the ranges to nothing when it
@@ -30,11 +31,15 @@ struct GpRangePlan {
     int launches = 0;
     bool timed_out = false;
 
-    GpRangePlan(uint64_t b, uint64_t e, int32_t slice_ms_req, double time_limit, double now)
+    // first_div: the first launch takes 1/first_div of the tasks (256 for the
+    // generic kernel; a kernel whose unit of work is a whole root passes a
+    // smaller divisor, since a short first range there holds a few heavy
+    // roots and leaves most waves idle)
+    GpRangePlan(uint64_t b, uint64_t e, int32_t slice_ms_req, double time_limit, double now, uint64_t first_div = 256)
         : begin(b), end(std::max(b, e)), cur(b), time_limit_s(time_limit), t_start(now) {
         const uint64_t n = end - begin;
         if (slice_ms_req > 0) slice_ms = (double)slice_ms_req;
-        size = (slice_ms_req < 0 || n <= (1ull << 20)) ? n : std::max<uint64_t>(1, n / 256);
+        size = (slice_ms_req < 0 || n <= (1ull << 20)) ? n : std::max<uint64_t>(1, n / std::max<uint64_t>(first_div, 1));
     }
     bool single() const { return size == end - begin && launches == 0; }
     // the next range to launch; false when every task has been launched or
