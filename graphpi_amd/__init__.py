@@ -171,6 +171,9 @@ _SIGS = {
     "gp_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(Tuning),
                                 ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int64),
                                 ctypes.POINTER(Stats)]),
+    "gp_count_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(Tuning),
+                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(Stats)]),
     "gp_count_tasks": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.POINTER(ctypes.c_uint64)]),
     "gp_count_footprint": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
                                           ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64)]),
@@ -444,6 +447,16 @@ class DeviceGraph:
         if rc == GP_ETIME:
             raise GPTimeout(rc, lib().gp_last_error().decode(errors="replace"), CountResult(None, raw.value, st))
         _check(rc)
+        return CountResult(cnt.value, raw.value, st)
+
+    def count_noalloc(self, sched, tuning=None):
+        """gp_count_noalloc: the same count with no device allocation (static
+        scratch, one launch)."""
+        raw = ctypes.c_uint64()
+        cnt = ctypes.c_int64()
+        st = Stats()
+        _check(lib().gp_count_noalloc(self._h, ctypes.byref(sched), ctypes.byref(tuning) if tuning else None,
+                                      ctypes.byref(raw), ctypes.byref(cnt), ctypes.byref(st)))
         return CountResult(cnt.value, raw.value, st)
 
     def count_batch(self, scheds, tuning=None):

@@ -97,7 +97,8 @@ struct KArgs {
     int32_t hash_lg;
     int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
     unsigned long long* heavy;     // deferred: plan << 56 | task << 24 | min(candidates, 2^24 - 1)
-    unsigned long long* heavy_cnt; // number deferred (may exceed GP_HEAVY_CAP)
+    unsigned long long* heavy_cnt; // number deferred (may exceed heavy_cap)
+    uint64_t heavy_cap;            // entries of the deferred list; the rest run in place
     unsigned long long* counter2;  // work counter of the parts launch
     unsigned long long* out; // per plan p: [3p] raw sum, [3p+1] B_alg bytes, [3p+2] tasks taken
     unsigned long long* clocks; // per wave: start, finish (wall clock ticks)
@@ -1466,7 +1467,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     // Work loop.  First launch (PARTS = false): chunks of tasks, deferring
     // heavy ones.  Second launch (PARTS = true): parts of the deferred tasks,
     // one per fetch.
-    const uint64_t n_items = !PARTS ? a.n_tasks : min((uint64_t)*a.heavy_cnt, (uint64_t)GP_HEAVY_CAP) * GP_SPLIT_PARTS;
+    const uint64_t n_items = !PARTS ? a.n_tasks : min((uint64_t)*a.heavy_cnt, a.heavy_cap) * GP_SPLIT_PARTS;
     uint64_t cur = 0, cend = 0;
     for (;;) {
         uint64_t t;
@@ -1557,7 +1558,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             unsigned long long slot = 0;
             if (lane == 0) slot = atomicAdd(a.heavy_cnt, 1ull);
             slot = __shfl(slot, 0);
-            if (slot < GP_HEAVY_CAP) {
+            if (slot < a.heavy_cap) {
                 if (lane == 0)
                     a.heavy[slot] = (unsigned long long)cur_plan << 56 | (unsigned long long)t << 24 |
                                     (unsigned long long)min(s_ll[2], 0xFFFFFF);
@@ -2333,6 +2334,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.heavy_cnt = c->d_cnt + 2;
     a.out = c->d_cnt + 3;
     a.heavy = c->d_heavy;
+    a.heavy_cap = GP_HEAVY_CAP;
     a.scratch = c->d_scratch;
     a.slot_stride = slot_stride;
     a.wave_stride = wave_stride;
@@ -2610,6 +2612,185 @@ extern "C" int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_
                         int64_t* count, gp_stats* stats) {
     if (!g || !sched) return gp_fail(GP_EINVAL, "null argument");
     return count_ranged(const_cast<gp_graph*>(g), sched, tuning, raw_sum, count, stats);
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free counting (gp_count_noalloc)
+// ---------------------------------------------------------------------------
+// Every device buffer the count needs besides the graph and the caller's
+// outputs (kernel plan and work layout, counters, clock stamps, the deferred
+// task list and the per-wave prefix sets) lives in one static device array of
+// GP_NOALLOC_BYTES, so the call makes no hipMalloc/hipFree.  Each device the
+// library runs on holds its own copy of the array for the life of the
+// process.  Calls on one device are serialised by a per-device lock, so they
+// never overlap on the array inside a process.
+#define GP_NOALLOC_BYTES (128ull << 20)   // static scratch per device
+#define GP_NOALLOC_HEAVY (1u << 16)       // deferred-task list entries (the rest run in place)
+#define GP_NOALLOC_DEVICES 64
+
+namespace {
+
+__device__ __align__(256) unsigned char scratch[GP_NOALLOC_BYTES];
+
+struct NoAllocDevice {
+    std::mutex mu;
+    unsigned char* ptr = nullptr; // device address of `scratch` on this device
+};
+NoAllocDevice g_noalloc[GP_NOALLOC_DEVICES];
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+} // namespace
+
+extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, const gp_tuning* tuning,
+                                uint64_t* raw_sum, int64_t* count, gp_stats* stats) {
+    if (!gc || !sched) return gp_fail(GP_EINVAL, "null argument");
+    gp_graph* g = const_cast<gp_graph*>(gc);
+    if (g->device < 0 || g->device >= GP_NOALLOC_DEVICES) return gp_fail(GP_ERANGE, "device index out of range");
+    HIP_TRY(hipSetDevice(g->device));
+    NoAllocDevice& nd = g_noalloc[g->device];
+    std::lock_guard<std::mutex> lk(nd.mu);
+    if (!nd.ptr) {
+        // the array's address on this device (the host-side name is only a stand-in)
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return gp_fail(e != hipSuccess ? (int)e : GP_ENODEV,
+                           std::string("static scratch address: ") + hipGetErrorString(e));
+        nd.ptr = static_cast<unsigned char*>(p);
+    }
+    // one launch (plus the deferred parts) on the generic kernel; the pair
+    // level uses its LDS map, since its dense arrays would not fit
+    gp_tuning t{};
+    if (tuning) t = *tuning;
+    t.pair_map = 1;
+    t.use_graph = 0;
+    t.no_special = 1;
+    Launch ln;
+    {
+        const int rc = prepare(g, sched, 1, &t, &ln);
+        if (rc != GP_OK) return rc;
+    }
+    const size_t plans_bytes = sizeof(KPlan);
+    const size_t meta_bytes = plans_bytes + sizeof(uint64_t) * 4;
+    const size_t n_cnt = 6;
+    const size_t off_cnt = align256(meta_bytes);
+    const size_t off_heavy = off_cnt + align256(n_cnt * sizeof(unsigned long long));
+    const size_t off_clk = off_heavy + align256(GP_NOALLOC_HEAVY * sizeof(unsigned long long));
+    const size_t wave_bytes = (size_t)ln.wave_stride * sizeof(int32_t);
+    const size_t per_wave = 4 * sizeof(unsigned long long) + wave_bytes;
+    if (off_clk + 512 + per_wave > GP_NOALLOC_BYTES)
+        return gp_fail(GP_ENOMEM, "gp_count_noalloc: one wave needs " + std::to_string(per_wave) +
+                                      " bytes of scratch (prefix sets sized by the second-largest degree) beyond " +
+                                      std::to_string(off_clk) + " fixed bytes; the static scratch holds " +
+                                      std::to_string(GP_NOALLOC_BYTES) + " bytes");
+    const int64_t fit = (int64_t)((GP_NOALLOC_BYTES - off_clk - 512) / per_wave);
+    const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(ln.waves, fit));
+    const size_t off_scr = align256(off_clk + 4 * sizeof(unsigned long long) * (size_t)waves);
+    if (off_scr + (size_t)waves * wave_bytes > GP_NOALLOC_BYTES)
+        return gp_fail(GP_ENOMEM, "gp_count_noalloc: layout exceeds the static scratch");
+    unsigned char* base = nd.ptr;
+    unsigned char meta[sizeof(KPlan) + 4 * sizeof(uint64_t)];
+    std::memcpy(meta, ln.kps.data(), plans_bytes);
+    const uint64_t layout[4] = {ln.task_off[0], ln.task_off[1], ln.task_first[0], ln.task_cnt[0]};
+    std::memcpy(meta + plans_bytes, layout, sizeof(layout));
+    KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.plans = reinterpret_cast<const KPlan*>(base);
+    a.nplans = 1;
+    a.task_off = reinterpret_cast<const uint64_t*>(base + plans_bytes);
+    a.task_first = reinterpret_cast<const uint64_t*>(base + plans_bytes + 2 * sizeof(uint64_t));
+    a.task_cnt = reinterpret_cast<const uint64_t*>(base + plans_bytes + 3 * sizeof(uint64_t));
+    a.vertex = g->d_vertex;
+    a.edge = g->d_edge;
+    a.src = g->d_src;
+    a.v_cnt = g->v_cnt;
+    a.chunk = ln.chunk;
+    a.n_tasks = ln.n_tasks;
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + off_cnt);
+    a.counter = cnt;
+    a.counter2 = cnt + 1;
+    a.heavy_cnt = cnt + 2;
+    a.out = cnt + 3;
+    a.heavy = reinterpret_cast<unsigned long long*>(base + off_heavy);
+    a.heavy_cap = GP_NOALLOC_HEAVY;
+    a.scratch = reinterpret_cast<int32_t*>(base + off_scr);
+    a.slot_stride = ln.slot_stride;
+    a.wave_stride = ln.wave_stride;
+    a.tmp_stride = ln.tmp_stride;
+    a.bslot_stride = ln.bslot_stride;
+    a.hash_slots = ln.hash_slots;
+    a.hash_lg = ln.hash_lg;
+    a.probe_pct = t.probe_pct > 0 ? t.probe_pct : 100;
+    a.bitmap_ratio = t.bitmap_ratio == 0 ? GP_BITMAP_RATIO : std::max(0, t.bitmap_ratio);
+    a.dense = nullptr;
+    a.dense_stride = 0;
+    unsigned long long* clk = reinterpret_cast<unsigned long long*>(base + off_clk);
+    a.clocks = clk;
+    const KPlan& kp = ln.kps[0];
+    const int ext = (!ln.faithful && kp.batched && kp.bl_base >= 0 && g->v_cnt < GP_FLAG_VMAX) ? GP_EXT_SEARCH
+                                                                                              : GP_EXT_NONE;
+    const bool parts = ln.any_split && ln.n_tasks > 0;
+    unsigned long long host_out[6] = {0, 0, 0, 0, 0, 0};
+    hipError_t e = hipMemcpyAsync(base, meta, meta_bytes, hipMemcpyHostToDevice, 0);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, 0, cnt, (int)n_cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && ln.n_tasks > 0) {
+        void* kargs[] = {&a};
+        e = hipLaunchKernel(count_kernel_fn(ln.faithful, false, false, ext), dim3((unsigned)waves), dim3(WAVE), kargs,
+                            ln.lds_bytes, 0);
+        if (e == hipSuccess && parts) {
+            KArgs a2 = a;
+            a2.clocks = clk + 2 * waves;
+            void* pargs[] = {&a2};
+            e = hipLaunchKernel(count_kernel_fn(false, true, false, ext), dim3((unsigned)waves), dim3(WAVE), pargs,
+                                ln.lds_bytes, 0);
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, cnt, sizeof(host_out), hipMemcpyDeviceToHost, 0);
+    std::vector<unsigned long long> clocks(ln.n_tasks > 0 ? 4 * (size_t)waves : 0);
+    if (e == hipSuccess && ln.n_tasks > 0)
+        e = hipMemcpyAsync(clocks.data(), clk, sizeof(unsigned long long) * (parts ? 4 : 2) * (size_t)waves,
+                           hipMemcpyDeviceToHost, 0);
+    if (e == hipSuccess) e = hipStreamSynchronize(0);
+    if (e != hipSuccess) return gp_fail((int)e, std::string("gp_count_noalloc: ") + hipGetErrorString(e));
+    const uint64_t sum = host_out[3];
+    if (raw_sum) *raw_sum = sum;
+    if (count) *count = (int64_t)sum / (sched->redundancy ? sched->redundancy : 1);
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->balg_bytes = ln.faithful ? host_out[4] + (kp.root_mode ? 8ull * ln.task_cnt[0]
+                                                                      : ln.task_first[0] == 0 ? 8ull * (uint64_t)g->v_cnt : 0)
+                                        : 0;
+        stats->tasks = host_out[5];
+        stats->waves = (int32_t)waves;
+        stats->kernel_variant = ln.faithful ? 1 : 0;
+        stats->split_tasks = parts ? std::min<uint64_t>(host_out[2], GP_NOALLOC_HEAVY) : 0;
+        stats->inplace_tasks = parts && host_out[2] > GP_NOALLOC_HEAVY ? host_out[2] - GP_NOALLOC_HEAVY : 0;
+        stats->launches = 1;
+        stats->finished = 1;
+        stats->tasks_total = ln.task_cnt[0];
+        if (ln.n_tasks > 0) {
+            // kernel time from the waves' wall-clock stamps (no events: creating
+            // one is a runtime allocation)
+            int rate_khz = 0;
+            if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess ||
+                rate_khz <= 0)
+                rate_khz = 100000;
+            unsigned long long t0 = ~0ull, t1 = 0;
+            for (int64_t w = 0; w < waves; ++w) {
+                t0 = std::min(t0, clocks[2 * w]);
+                unsigned long long f = clocks[2 * w + 1];
+                if (parts) f = std::max(f, clocks[2 * waves + 2 * w + 1]);
+                t1 = std::max(t1, f);
+            }
+            stats->kernel_ms = t1 > t0 ? (double)(t1 - t0) / rate_khz : 0.0;
+            stats->finish_max_ms = stats->kernel_ms;
+        }
+    }
+    return GP_OK;
 }
 
 extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,

@@ -183,6 +183,21 @@ GP_API int gp_graph_bytes(const gp_graph* g, uint64_t* bytes);
 GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
                     uint64_t* raw_sum, int64_t* count, gp_stats* stats);
 
+/* gp_count without allocating device memory, for callers that cannot call
+ * hipMalloc (code under stream capture, tight serving loops, harnesses that
+ * forbid it).  No counterpart in the reference.  Every buffer the count needs
+ * besides the uploaded graph and the caller's outputs lives in one static
+ * device array of 128 MiB per device, held for the life of the process; the
+ * count runs as one launch over the tuning's task range (gp_count with
+ * slice_ms < 0) on the null stream, on the generic kernel, with fewer
+ * persistent waves when the per-wave prefix sets would not fit, at most 2^16
+ * deferred tasks (the rest run in place) and the last two loop levels on the
+ * LDS count map.  Results equal gp_count's.  A schedule whose per-wave sets
+ * alone exceed the array is rejected with GP_ENOMEM.  Calls on one device
+ * are serialised inside the library (they share the array). */
+GP_API int gp_count_noalloc(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
+                            uint64_t* raw_sum, int64_t* count, gp_stats* stats);
+
 /* Count n_scheds schedules over one uploaded graph with persistent launches
  * shared by the schedules: the schedules are grouped by the LDS their kernel
  * plan needs (at most four groups), and within a group the work items of
