@@ -2660,10 +2660,12 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
         nd.ptr = static_cast<unsigned char*>(p);
     }
     // one launch (plus the deferred parts) on the generic kernel; the pair
-    // level uses its LDS map, since its dense arrays would not fit
+    // level uses its LDS map, since its dense arrays would not fit, and keeps
+    // the ordinary task order (grouping by v1 serves the dense arrays)
     gp_tuning t{};
     if (tuning) t = *tuning;
     t.pair_map = 1;
+    t.no_pair_v1 = 1;
     t.use_graph = 0;
     t.no_special = 1;
     Launch ln;

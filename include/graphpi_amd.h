@@ -192,7 +192,7 @@ GP_API int gp_count(const gp_graph* g, const gp_sched_flat* sched, const gp_tuni
  * slice_ms < 0) on the null stream, on the generic kernel, with fewer
  * persistent waves when the per-wave prefix sets would not fit, at most 2^16
  * deferred tasks (the rest run in place) and the last two loop levels on the
- * LDS count map.  Results equal gp_count's.  A schedule whose per-wave sets
+ * LDS count map in the ordinary task order.  Results equal gp_count's.  A schedule whose per-wave sets
  * alone exceed the array is rejected with GP_ENOMEM.  Calls on one device
  * are serialised inside the library (they share the array). */
 GP_API int gp_count_noalloc(const gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning,
