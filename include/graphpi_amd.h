@@ -115,7 +115,8 @@ typedef struct gp_tuning {
                                count and deferred-parts launches, result read-back) as one
                                hipGraph, kept per call context and replayed with updated
                                kernel arguments while the launch shape repeats; 0 = stream
-                               launches.  Counts are identical either way */
+                               launches.  Counts are identical either way.  Covers the generic
+                               kernel; the House kernel (no_special = 0) runs as stream launches */
     int32_t no_shared_search; /* 1 = batched leaf level: search every block's set on its own
                                  instead of one flagged search of a containing set */
     int32_t no_pair_v1;     /* 1 = pair level: stream both sides per task instead of grouping

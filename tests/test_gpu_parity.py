@@ -23,6 +23,17 @@ def _tc(host):
     return ob.triangle_counting(host)
 
 
+def _generic(**kw):
+    """A tuning that keeps every count on the generic kernel (House under the
+    GraphPi schedule would otherwise take its specialised two-hop kernel), so
+    that the generic kernel's paths named by kw are the ones compared."""
+    t = gp.Tuning()
+    t.no_special = 1
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
 def _sched(host, adj, pm, rt, iep, tri=None):
     tri = _tc(host) if tri is None else tri
     s, valid = gp.build_schedule(adj, pm, rt, iep, max(host.v_cnt, 1), max(host.e_cnt, 1), tri)
@@ -123,14 +134,13 @@ def test_batched_level_matches_per_candidate(rmat13_path):
     host = gp.HostGraph.load_text(rmat13_path)
     dev = gp.DeviceGraph(host)
     tri = _tc(host)
-    tun = gp.Tuning()
-    tun.no_batch = 1
+    tun = _generic(no_batch=1)
     cases = [(name, st) for name in ["triangle", "wedge", "rectangle", "qg3", "house", "4clique", "5clique"]
              for st in [(1, 1, 1), (0, 1, 0), (1, 1, 0)]]
     cases += [("pentagon", (1, 1, 1)), ("cycle_6_tri", (1, 1, 1))]
     for name, (pm, rt, iep) in cases:
         s = _sched(host, gp.PATTERNS[name], pm, rt, iep, tri)
-        a, b = dev.count(s), dev.count(s, tun)
+        a, b = dev.count(s, _generic()), dev.count(s, tun)
         print(f"{name} ({pm},{rt},{iep}) batched {a.stats.kernel_ms:.2f} ms per-candidate "
               f"{b.stats.kernel_ms:.2f} ms", flush=True)
         assert a.raw == b.raw, (name, pm, rt, iep)
@@ -141,17 +151,14 @@ def test_pair_level_matches(name):
     host = gp.HostGraph.rmat(11, 8, seed=7)
     dev = gp.DeviceGraph(host)
     tri = _tc(host)
-    no_pair = gp.Tuning()
-    no_pair.no_pair = 1
-    no_batch = gp.Tuning()
-    no_batch.no_batch = 1
-    lds_map = gp.Tuning()  # the pair level's LDS count map instead of the dense array
-    lds_map.pair_map = 1
+    no_pair = _generic(no_pair=1)
+    no_batch = _generic(no_batch=1)
+    lds_map = _generic(pair_map=1)  # the pair level's LDS count map instead of the dense array
     for pm, rt, iep in [(1, 1, 1), (1, 1, 0), (0, 1, 0)]:
         s, valid = gp.build_schedule(gp.PATTERNS[name], pm, rt, iep, host.v_cnt, host.e_cnt, tri)
         if not valid:
             continue
-        a, b, c, m = dev.count(s), dev.count(s, no_pair), dev.count(s, no_batch), dev.count(s, lds_map)
+        a, b, c, m = dev.count(s, _generic()), dev.count(s, no_pair), dev.count(s, no_batch), dev.count(s, lds_map)
         print(f"{name} ({pm},{rt},{iep}) pair dense {a.stats.kernel_ms:.2f} ms, LDS map {m.stats.kernel_ms:.2f} ms, "
               f"batched {b.stats.kernel_ms:.2f} ms, per-candidate {c.stats.kernel_ms:.2f} ms", flush=True)
         assert a.raw == b.raw == c.raw == m.raw, (name, pm, rt, iep)
@@ -190,11 +197,8 @@ def test_split_heavy_tasks_same_counts(rmat13_path):
     tri = _tc(host)
     tuns = {}
     for v in (1, 40, -1):
-        tuns[v] = gp.Tuning()
-        tuns[v].split_min = v
-    no_batch = gp.Tuning()
-    no_batch.no_batch = 1
-    no_batch.split_min = 1
+        tuns[v] = _generic(split_min=v)
+    no_batch = _generic(no_batch=1, split_min=1)
     cases = [(name, (1, 1, 1)) for name in ["triangle", "rectangle", "house", "4clique", "5clique", "pentagon",
                                              "qg3", "cycle_6_tri", "5path"]]
     cases += [(name, (0, 1, 0)) for name in ["house", "pentagon", "4clique"]]
@@ -207,7 +211,7 @@ def test_split_heavy_tasks_same_counts(rmat13_path):
             print(f"{name} ({pm},{rt},{iep}) split_min={v}: {r.stats.split_tasks} split tasks, "
                   f"{r.stats.kernel_ms:.2f} ms vs {base.stats.kernel_ms:.2f} ms", flush=True)
             assert r.raw == base.raw, (name, pm, rt, iep, v)
-        assert dev.count(s).raw == base.raw
+        assert dev.count(s).raw == dev.count(s, _generic()).raw == base.raw
         assert dev.count(s, no_batch).raw == base.raw, (name, "no_batch")
     # with task ranges: the deferred tasks of each range stay in that range
     s = _sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)
@@ -218,11 +222,10 @@ def test_no_truncation_same_counts(rmat13_path):
     host = gp.HostGraph.load_text(rmat13_path)
     dev = gp.DeviceGraph(host)
     tri = _tc(host)
-    tun = gp.Tuning()
-    tun.no_truncate = 1
+    tun = _generic(no_truncate=1)
     for name in ["4clique", "5clique", "house", "tri_edge_tri"]:
         s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
-        assert dev.count(s, tun).raw == dev.count(s).raw, name
+        assert dev.count(s, tun).raw == dev.count(s, _generic()).raw == dev.count(s).raw, name
 
 
 @pytest.mark.parametrize("wpc,chunk", [(1, 1), (4, 3), (32, 64)])
@@ -233,9 +236,10 @@ def test_launch_shapes_same_counts(rmat13_path, wpc, chunk):
     tun = gp.Tuning()
     tun.waves_per_cu = wpc
     tun.chunk = chunk
+    gen = _generic(waves_per_cu=wpc, chunk=chunk)
     for name in ["house", "4clique"]:
         s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
-        assert dev.count(s, tun).count == dev.count(s).count
+        assert dev.count(s, tun).count == dev.count(s, gen).count == dev.count(s).count
 
 
 def _perm_count(n, k):
@@ -436,21 +440,21 @@ def test_shared_search_matches_per_block_search():
     assert host.max_degree >= 4096
     dev = gp.DeviceGraph(host)
     tri = _tc(host)
-    off = gp.Tuning()
-    off.no_shared_search = 1
+    off = _generic(no_shared_search=1)
     n = shared = 0
     for k in (5, 6):
         for i, adj in enumerate(gp.motifs(k)):
             s = _sched(host, adj, 1, 1, 1, tri)
             shared += gp.kernel_plan(s)["shared_search"]
-            a, b = dev.count(s), dev.count(s, off)
-            assert a.raw == b.raw, (k, i)
+            a, b = dev.count(s, _generic()), dev.count(s, off)
+            assert a.raw == b.raw == dev.count(s).raw, (k, i)
             n += 1
     assert n == 21 + 112 and shared > 0
     s = _sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)
     want = ob.pattern_matching(host, s, threads=8)
-    got = dev.count(s)
-    assert (got.raw, got.count) == (want["raw"], want["count"])
+    for t in (None, _generic()):
+        got = dev.count(s, t)
+        assert (got.raw, got.count) == (want["raw"], want["count"])
 
 
 @pytest.mark.parametrize("graph", ["rmat", "chunglu"])
@@ -498,17 +502,13 @@ def test_bitmap_staging_matches():
     host = gp.HostGraph.rmat(12, 12, 0.57, 0.19, 0.19, 7)
     dev = gp.DeviceGraph(host)
     tri = _tc(host)
-    off = gp.Tuning()
-    off.bitmap_ratio = -1
-    wide = gp.Tuning()
-    wide.bitmap_ratio = 1 << 20
-    noshare = gp.Tuning()
-    noshare.no_shared_search = 1
-    noshare.bitmap_ratio = 1 << 20
+    off = _generic(bitmap_ratio=-1)
+    wide = _generic(bitmap_ratio=1 << 20)
+    noshare = _generic(no_shared_search=1, bitmap_ratio=1 << 20)
     for name in ["triangle", "4clique", "5clique", "house", "qg3", "rectangle", "cycle_6_tri", "hourglass"]:
         s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
         want = ob.pattern_matching(host, s, threads=8)
-        for t in (None, off, wide, noshare):
+        for t in (None, _generic(), off, wide, noshare):
             got = dev.count(s, t)
             assert (got.raw, got.count) == (want["raw"], want["count"]), name
 
@@ -525,12 +525,12 @@ def test_sampled_index_matches():
     s = _sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)
     assert gp.kernel_plan(s)["shared_search"]
     want = ob.pattern_matching(host, s, threads=8)
-    assert dev.count(s).raw == want["raw"]
+    assert dev.count(s).raw == dev.count(s, _generic()).raw == want["raw"]
     for stage in (32, 100):
-        small = gp.Tuning()
-        small.lds_stage = stage
+        small = _generic(lds_stage=stage)
         got = dev.count(s, small)
         assert (got.raw, got.count) == (want["raw"], want["count"]), stage
+
 
 @pytest.mark.parametrize("graph", ["rmat13", "hubs", "chunglu", "wiki"])
 def test_house_twohop_kernel_matches(graph, rmat13_path, wiki_vote_path):
