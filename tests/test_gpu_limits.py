@@ -105,12 +105,14 @@ def test_deferred_list_overflow_runs_in_place():
     over = gp.Tuning()
     over.split_min = 1
     over.slice_ms = -1
+    over.no_special = 1  # the generic kernel (House's own kernel defers nothing)
     r = dev.count(s, over)
     print(f"deferred {r.stats.split_tasks}, in place {r.stats.inplace_tasks}, {r.stats.kernel_ms:.1f} ms", flush=True)
     assert r.stats.split_tasks == 1 << 20 and r.stats.inplace_tasks > 100000
     never = gp.Tuning()
     never.split_min = -1
     never.slice_ms = -1
+    never.no_special = 1
     base = dev.count(s, never)
     assert base.stats.split_tasks == 0
     assert r.raw == base.raw == dev.count(s).raw
@@ -124,9 +126,11 @@ def test_bounded_launches_equal_one_launch():
         > 1 << 20
     one = gp.Tuning()
     one.slice_ms = -1
-    for name, ms in [("house", 1), ("4clique", 5), ("pentagon", 1), ("rectangle", 5)]:
+    for name, ms, special in [("house", 1, 0), ("house", 1, 1), ("4clique", 5, 0), ("pentagon", 1, 0),
+                              ("rectangle", 5, 0)]:
         tiny = gp.Tuning()
         tiny.slice_ms = ms  # launches of about ms each (at least 1/1024 of the tasks)
+        tiny.no_special = special
         s, _ = gp.build_schedule(gp.PATTERNS[name], 1, 1, True, host.v_cnt, host.e_cnt, tri)
         a, b, c = dev.count(s, tiny), dev.count(s, one), dev.count(s)
         assert a.stats.launches <= 1300
