@@ -247,6 +247,9 @@ def main():
     ap.add_argument("--once", default=None,
                     help="comma-separated patterns counted once outside the steps ('' for none)")
     ap.add_argument("--scale", type=int, default=None, help="override the R-MAT scale (exploration only)")
+    ap.add_argument("--oriented", type=int, default=None, choices=(0, 1, 2),
+                    help="relabel the generated graph by degree as the loader's oriented_type does "
+                         "(1 = high degree first; default: the configuration's choice)")
     ap.add_argument("--sample-frac", type=float, default=0.0, metavar="F",
                     help="exploration only: count 64 task ranges spread over the task space, covering a fraction F "
                          "of the tasks, and print the full count's estimated kernel time (no count is reported)")
@@ -287,6 +290,10 @@ def main():
         host = gp.HostGraph.chung_lu(scale, ef, 2.1, 30000.0, seed)
     else:
         host = gp.HostGraph.rmat(scale, ef, 0.57, 0.19, 0.19, seed)
+    oriented = cfg.get("oriented", 0) if args.oriented is None else args.oriented
+    if oriented:
+        host = host.reorder(oriented)
+        cfg["workload"] += f"; relabelled {'high' if oriented == 1 else 'low'} degree first (oriented_type {oriented})"
     log(f"graph: {host.v_cnt} vertices, {host.e_cnt // 2} undirected edges, max degree {host.max_degree} "
         f"({time.perf_counter() - t0:.1f} s)")
     dev = gp.DeviceGraph(host)
@@ -318,7 +325,7 @@ def main():
 
     if args.sample_frac > 0:
         est = {}
-        for name in cfg["patterns"]:
+        for name in cfg["patterns"] + [n for n in cfg["once"] if n not in cfg["patterns"]]:
             total, parts = totals[name], 64
             width = max(1, int(total * args.sample_frac / parts))
             tun, ms, sampled = gp.Tuning(), 0.0, 0

@@ -14,9 +14,12 @@
 // Chung-Lu (gen = 1): n vertices with expected degrees w_i drawn from a
 // Pareto law with tail exponent alpha (w = u^(-1/(alpha-1)), u in (0,1]),
 // scaled so that sum(w) = 2 * target_edges and capped at max_expected_degree
-// (the scale is recomputed after capping, 8 rounds); then target_edges
-// samples, each endpoint drawn independently with probability w_i / sum(w)
-// (inverse CDF by binary search).  Self-loops and duplicates are dropped.
+// (the scale is recomputed after capping, 8 rounds); then samples, each
+// endpoint drawn independently with probability w_i / sum(w) (inverse CDF by
+// binary search), self-loops and duplicates dropped, in rounds that continue
+// the sample stream until at least target_edges distinct edges exist (the
+// first round draws target_edges samples, each later one the shortfall plus
+// an eighth).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -130,18 +133,28 @@ int chung_lu(int64_t n, int64_t target_edges, double alpha, double max_w, uint64
         cdf[i] = s;
     }
     const uint64_t base2 = splitmix64(seed ^ 0xFEEDull);
-    std::vector<uint64_t> und((size_t)target_edges);
+    std::vector<uint64_t> und;
+    uint64_t drawn = 0;  // samples taken from the stream so far
+    for (int round = 0; round < 64 && und.size() < (size_t)target_edges; ++round) {
+        const uint64_t short_by = (uint64_t)target_edges - und.size();
+        const uint64_t m = round == 0 ? short_by : short_by + short_by / 8 + 16;
+        const size_t old = und.size();
+        und.resize(old + (size_t)m);
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < target_edges; ++i) {
-        double r1 = u01(splitmix64(base2 + 2 * (uint64_t)i)) * s;
-        double r2 = u01(splitmix64(base2 + 2 * (uint64_t)i + 1)) * s;
-        uint32_t u = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), r1) - cdf.begin());
-        uint32_t v = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), r2) - cdf.begin());
-        if (u >= (uint32_t)n) u = (uint32_t)n - 1;
-        if (v >= (uint32_t)n) v = (uint32_t)n - 1;
-        und[i] = u == v ? ~0ull : ((uint64_t)std::min(u, v) << 32 | std::max(u, v));
+        for (int64_t j = 0; j < (int64_t)m; ++j) {
+            const uint64_t i = drawn + (uint64_t)j;
+            double r1 = u01(splitmix64(base2 + 2 * i)) * s;
+            double r2 = u01(splitmix64(base2 + 2 * i + 1)) * s;
+            uint32_t u = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), r1) - cdf.begin());
+            uint32_t v = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), r2) - cdf.begin());
+            if (u >= (uint32_t)n) u = (uint32_t)n - 1;
+            if (v >= (uint32_t)n) v = (uint32_t)n - 1;
+            und[old + (size_t)j] = u == v ? ~0ull : ((uint64_t)std::min(u, v) << 32 | std::max(u, v));
+        }
+        drawn += m;
+        gp_sort_unique(und);
+        if (!und.empty() && und.back() == ~0ull) und.pop_back(); // the self-loop marker sorts last
     }
-    und.erase(std::remove(und.begin(), und.end(), ~0ull), und.end());
     return finish(und, out);
 }
 
