@@ -22,7 +22,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 HOST_SOURCES = ["host/pattern.cpp", "host/schedule.cpp", "host/dataloader.cpp", "host/gen.cpp",
                 "host/graph.cpp", "host/capi.cpp", "host/kplan.cpp"]
-HIP_SOURCES = ["hip/count.hip", "hip/house.hip"]
+HIP_SOURCES = ["hip/count.hip", "hip/special.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX_HOST", "g++")

@@ -35,7 +35,7 @@
 
 #include "../gp_internal.h"
 #include "../range_plan.h"
-#include "house.h"
+#include "special.h"
 #include "kplan.h"
 #include "schedule.h"
 
@@ -1692,6 +1692,7 @@ struct gp_graph {
     int32_t* d_src = nullptr;
     int32_t* d_sup = nullptr;   // per edge slot |N(src) ∩ N(dst)|, built on first use
     double sup_build_ms = 0;    // time of that build
+    uint64_t* d_vtri = nullptr; // per vertex: triangles containing it, built on first use
     std::mutex mu;
     std::mutex sup_mu;
     std::vector<CallCtx*> idle;
@@ -2143,6 +2144,10 @@ int ensure_support(gp_graph* g, CallCtx* c) {
     return GP_OK;
 }
 
+int run_tri_edge_tri(gp_graph* g, const gp_sched_flat* sched, uint64_t* raw, int64_t* count, gp_stats* stats);
+int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw_sum,
+                 int64_t* count, gp_stats* stats);
+
 // House under the GraphPi schedule's order, with the specialised two-hop
 // kernel (house.hip): one launch over a task range of edge slots.
 int run_house(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, bool all_v1, uint64_t* raw,
@@ -2263,6 +2268,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         const int shape = gp_sched_shape(&scheds[0]);
         if (shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL)
             return run_house(g, scheds, tuning, shape == GP_SHAPE_HOUSE_ALL, raw, count, stats);
+        if (shape == GP_SHAPE_TRI_EDGE_TRI && !(tuning && (tuning->task_begin != 0 || tuning->task_end > 0)))
+            return run_tri_edge_tri(g, scheds, raw, count, stats);
     }
     Launch ln;
     {
@@ -2460,6 +2467,101 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     return GP_OK;
 }
 
+// Two triangles joined by an edge under the GraphPi schedule (shape
+// GP_SHAPE_TRI_EDGE_TRI), counted in closed form instead of by walking every
+// 3-path.  An occurrence is a bridge edge {a, b} with a triangle T1 at a that
+// avoids b and a triangle T2 at b that avoids a, all six vertices distinct:
+//   C = sum over edges {a, b} of (t_a - s_ab)(t_b - s_ab) - X_ab,
+// with t the triangles at a vertex, s the edge supports, and X_ab the pairs
+// (T1, T2) that share a vertex.  By inclusion-exclusion over the shared
+// vertices v (each in N(a) ∩ N(b), where T1 ∋ v has s_av - 1 choices and
+// T2 ∋ v has s_bv - 1), pairs sharing both vertices being the edges inside
+// N(a) ∩ N(b), i.e. the 4-cliques on {a, b}:
+//   X_ab = sum_{v in N(a) ∩ N(b)} (s_av - 1)(s_bv - 1) - K4(a, b),
+// and the K4 terms add up to 6 x the number of 4-cliques.  The reference's
+// loop nest enforces only v3 < v0 (its other two restrictions sit at IEP
+// depths), which keeps exactly 4 of each occurrence's 8 labelled embeddings,
+// so its raw sum is 4 C (mod 2^64) and its result raw / 4.  Everything is u64
+// arithmetic, so even a count past 2^63 matches the reference's wrapped sum.
+int run_tri_edge_tri(gp_graph* g, const gp_sched_flat* sched, uint64_t* raw, int64_t* count, gp_stats* stats) {
+    HIP_TRY(hipSetDevice(g->device));
+    // 4-cliques, each once (v3 < v2 < v1 < v0), on the generic kernel in bounded launches
+    const int adj4[16] = {0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0, 1, 1, 1, 1, 0};
+    Schedule s4(adj4, 4);
+    s4.add_restrict({{0, 1}, {1, 2}, {2, 3}});
+    gp_sched_flat flat4;
+    int rc = s4.flatten(&flat4);
+    if (rc != GP_OK) return gp_fail(rc, "4-clique schedule");
+    uint64_t k4 = 0;
+    gp_stats st4{};
+    rc = count_ranged(g, &flat4, nullptr, &k4, nullptr, &st4);
+    if (rc != GP_OK) return rc;
+    CallCtx* c = nullptr;
+    rc = acquire_ctx(g, 64, &c);
+    if (rc != GP_OK) return rc;
+    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        destroy_ctx(c);
+        return gp_fail(GP_ENOMEM, std::string("call buffers: ") + hipGetErrorString(e));
+    }
+    const bool sup_now = g->d_sup == nullptr;
+    rc = ensure_support(g, c);
+    if (rc != GP_OK) {
+        destroy_ctx(c);
+        return rc;
+    }
+    float ms_t = 0, ms_e = 0;
+    {
+        std::lock_guard<std::mutex> lk(g->sup_mu);
+        if (!g->d_vtri && g->v_cnt > 0) {
+            uint64_t* t = nullptr;
+            e = hipMalloc((void**)&t, sizeof(uint64_t) * (size_t)g->v_cnt);
+            if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+            if (e == hipSuccess) e = gp_vertex_triangles_launch(g->d_vertex, g->d_sup, g->v_cnt, t, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms_t, c->e0, c->e1);
+            if (e != hipSuccess) {
+                if (t) (void)hipFree(t);
+                destroy_ctx(c);
+                return gp_fail((int)e, std::string("vertex triangles: ") + hipGetErrorString(e));
+            }
+            g->d_vtri = t;
+        }
+    }
+    unsigned long long host_out[8] = {0};
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, 8);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
+    if (e == hipSuccess && g->e_cnt)
+        e = gp_tri_edge_launch(g->d_vertex, g->d_edge, g->d_src, g->d_sup, g->d_vtri, g->e_cnt, c->d_cnt + 1,
+                               c->d_cnt + 3, (unsigned)(g->num_cu * 16), c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms_e, c->e0, c->e1);
+    if (e != hipSuccess) {
+        destroy_ctx(c);
+        return gp_fail((int)e, std::string("tri-edge-tri kernel: ") + hipGetErrorString(e));
+    }
+    release_ctx(g, c);
+    const uint64_t cnt = host_out[3] + 6ull * k4;       // C mod 2^64
+    const uint64_t r = cnt * (uint64_t)sched->redundancy; // the loop nest's raw sum, 4 C
+    if (raw) *raw = r;
+    if (count) *count = (int64_t)r / sched->redundancy;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->kernel_ms = st4.kernel_ms + ms_t + ms_e + (sup_now ? g->sup_build_ms : 0.0);
+        stats->tasks = g->e_cnt;
+        stats->tasks_total = g->e_cnt;
+        stats->waves = g->num_cu * 16;
+        stats->kernel_variant = 3;
+        stats->launches = st4.launches + 1;
+        stats->finished = 1;
+    }
+    return GP_OK;
+}
+
 } // namespace
 
 extern "C" int gp_device_count(int32_t* n) {
@@ -2532,13 +2634,15 @@ extern "C" int gp_graph_free(gp_graph* g) {
     if (g->d_edge) (void)hipFree(g->d_edge);
     if (g->d_src) (void)hipFree(g->d_src);
     if (g->d_sup) (void)hipFree(g->d_sup);
+    if (g->d_vtri) (void)hipFree(g->d_vtri);
     delete g;
     return GP_OK;
 }
 
 extern "C" int gp_graph_bytes(const gp_graph* g, uint64_t* bytes) {
     if (!g || !bytes) return gp_fail(GP_EINVAL, "null argument");
-    *bytes = sizeof(uint32_t) * ((uint64_t)g->v_cnt + 1) + (g->d_sup ? 3 : 2) * sizeof(int32_t) * (uint64_t)g->e_cnt;
+    *bytes = sizeof(uint32_t) * ((uint64_t)g->v_cnt + 1) + (g->d_sup ? 3 : 2) * sizeof(int32_t) * (uint64_t)g->e_cnt +
+             (g->d_vtri ? sizeof(uint64_t) * (uint64_t)g->v_cnt : 0);
     return GP_OK;
 }
 
@@ -2565,6 +2669,8 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
     const uint64_t end = t.task_end > 0 ? std::min<uint64_t>((uint64_t)t.task_end, all_tasks) : all_tasks;
     // the House kernel's unit of work is a whole root, the generic kernel's an edge slot
     const int shape = (t.collect_balg || t.no_special) ? GP_SHAPE_GENERIC : gp_sched_shape(sched);
+    if (shape == GP_SHAPE_TRI_EDGE_TRI && t.task_begin == 0 && t.task_end <= 0)
+        return run_tri_edge_tri(g, sched, raw_sum, count, stats);
     const bool per_root = shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL;
     GpRangePlan plan(begin, end, t.slice_ms, t.time_limit_s, wall_seconds(), per_root ? 4 : 256);
     if (plan.single()) return run_plans(g, sched, 1, &t, raw_sum, count, stats);

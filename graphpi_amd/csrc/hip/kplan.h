@@ -85,6 +85,9 @@ struct KPlan {
 #define GP_SHAPE_HOUSE 1     // House in the GraphPi order: v1 < v0 in N(v0), v2 in N(v0), IEP over
                              // v3 in N(v0) ∩ N(v1) and v4 in N(v1) ∩ N(v2)
 #define GP_SHAPE_HOUSE_ALL 2 // the same loop nest without the restriction v1 < v0
+#define GP_SHAPE_TRI_EDGE_TRI 3 // two triangles joined by an edge in the GraphPi order: loops over the
+                                // 3-path v0-v1-v2-v3 with v3 < v0, IEP over v4 in N(v2) ∩ N(v3) and
+                                // v5 in N(v0) ∩ N(v1); redundancy 4 (raw sum = 4 x the count)
 int gp_sched_shape(const gp_sched_flat* s);
 
 // Returns GP_OK or an error code (with gp_fail message) for unsupported input.

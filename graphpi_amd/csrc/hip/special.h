@@ -1,5 +1,7 @@
-// Launch interface of the House-specialised kernel (house.hip) and of the
-// edge-support kernel it reads.
+// Launch interfaces of the schedule-specialised kernels (special.hip): House
+// counted per root with two-hop counts, the edge-support kernel both shapes
+// read, and the per-edge sum of the "two triangles joined by an edge" closed
+// form.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,3 +25,12 @@ struct GpHouseArgs {
 hipError_t gp_house_launch(const GpHouseArgs& args, unsigned waves, hipStream_t stream);
 hipError_t gp_support_launch(const uint32_t* vertex, const int32_t* edge, const int32_t* src, uint64_t e_cnt,
                              int32_t* sup, unsigned long long* counter, unsigned waves, hipStream_t stream);
+
+// t[v] = triangles containing v = (sum of sup over v's edge slots) / 2
+hipError_t gp_vertex_triangles_launch(const uint32_t* vertex, const int32_t* sup, int32_t v_cnt, uint64_t* t,
+                                      hipStream_t stream);
+// out[0] += sum over edges {a, b} (a > b) of (t_a - s)(t_b - s) - sum_{v in N(a) ∩ N(b)} (s_av - 1)(s_bv - 1),
+// s = sup(a, b), all in u64 (two's-complement wrap-around)
+hipError_t gp_tri_edge_launch(const uint32_t* vertex, const int32_t* edge, const int32_t* src, const int32_t* sup,
+                              const uint64_t* t, uint64_t e_cnt, unsigned long long* counter, unsigned long long* out,
+                              unsigned waves, hipStream_t stream);

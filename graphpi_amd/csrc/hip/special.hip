@@ -1,5 +1,15 @@
-// Schedule-specialised kernel: House (5 vertices, 6 edges) under the
-// GraphPi schedule's vertex order, counted per root with two-hop counts.
+// Schedule-specialised kernels.  Each computes exactly the raw sum the
+// reference's loop nest computes for one schedule shape, by a different
+// grouping of the same integer sum:
+//   - house_twohop_kernel: House under the GraphPi order, per root with
+//     two-hop counts (below);
+//   - support_kernel: sup(e) = |N(u) ∩ N(v)| for every edge slot (u, v);
+//   - vertex_triangles_kernel and tri_edge_kernel: the closed form of "two
+//     triangles joined by an edge" (the host adds 6 x the 4-clique count;
+//     see run_tri_edge_tri in count.hip).
+//
+// House (5 vertices, 6 edges) under the GraphPi schedule's vertex order,
+// counted per root with two-hop counts.
 //
 // The reference evaluates House (src/graph.cpp:247-398) as: v0; v1 in N(v0)
 // with v1 < v0; v2 in N(v0); then an IEP leaf over v3 in A = N(v0) ∩ N(v1)
@@ -27,9 +37,10 @@
 // level (on R-MAT scale 22: 4.1e11 against 3.9e13 elements).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
-#include "house.h"
+#include "special.h"
 
 #define WAVE 64
 #define GP_UNROLL 4
@@ -321,7 +332,119 @@ __global__ void __launch_bounds__(WAVE) support_kernel(const uint32_t* __restric
     }
 }
 
+__global__ void vertex_triangles_kernel(const uint32_t* __restrict__ vertex, const int32_t* __restrict__ sup,
+                                        int32_t v_cnt, uint64_t* __restrict__ t) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < v_cnt;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t s = 0;
+        for (uint32_t e = vertex[v]; e < vertex[v + 1]; ++e) s += (uint32_t)sup[e];
+        t[v] = s / 2;
+    }
+}
+
+// Per edge {a, b} (the slot with a = src > b = dst): (t_a - s)(t_b - s) minus
+// the sum over common neighbours v of (sup(a, v) - 1)(sup(b, v) - 1).  The
+// common neighbours come with their slots in both lists: the shorter list's
+// elements are searched in the longer one, and the search's position is the
+// slot.  Lanes take one edge each when the shorter list has at most 8
+// elements, the whole wave otherwise.
+__global__ void __launch_bounds__(WAVE) tri_edge_kernel(const uint32_t* __restrict__ vertex,
+                                                        const int32_t* __restrict__ edge,
+                                                        const int32_t* __restrict__ src,
+                                                        const int32_t* __restrict__ sup,
+                                                        const uint64_t* __restrict__ t, uint64_t e_cnt,
+                                                        unsigned long long* counter, unsigned long long* out) {
+    const int lane = lane_id();
+    const gcptr E = (gcptr)edge;
+    const gcptr SUP = (gcptr)sup;
+    uint64_t acc = 0; // per lane, u64 wrap-around
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(counter, (unsigned long long)WAVE);
+        base = first_lane_u64(base);
+        if (base >= e_cnt) break;
+        const uint64_t end = min((uint64_t)base + WAVE, e_cnt);
+        const uint64_t e = base + (uint64_t)lane;
+        int32_t a = 0, b = 0, da = 0, db = 0;
+        uint32_t la = 0, lb = 0;
+        bool mine = false;
+        if (e < end) {
+            a = src[e];
+            b = E[e];
+            mine = a > b;
+            if (mine) {
+                la = vertex[a];
+                da = (int32_t)(vertex[a + 1] - la);
+                lb = vertex[b];
+                db = (int32_t)(vertex[b + 1] - lb);
+                const uint64_t s = (uint32_t)SUP[e];
+                acc += (t[a] - s) * (t[b] - s);
+            }
+        }
+        const bool small = mine && min(da, db) <= 8;
+        if (small) {
+            const bool a_short = da <= db;
+            const uint32_t ls = a_short ? la : lb, ll = a_short ? lb : la;
+            const int32_t ns = min(da, db), nl = max(da, db);
+            for (int32_t i = 0; i < ns; ++i) {
+                const int32_t x = E[ls + i];
+                int32_t lo = 0, len = nl;
+                while (len > 0) {
+                    const int32_t half = len >> 1;
+                    const bool right = E[ll + lo + half] < x;
+                    lo = right ? lo + half + 1 : lo;
+                    len = right ? len - half - 1 : half;
+                }
+                if (lo < nl && E[ll + lo] == x)
+                    acc -= (uint64_t)(uint32_t)(SUP[ls + i] - 1) * (uint64_t)(uint32_t)(SUP[ll + lo] - 1);
+            }
+        }
+        uint64_t big = __ballot(mine && !small);
+        while (big) {
+            const int j = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const int32_t ja = __shfl(da, j), jb = __shfl(db, j);
+            const uint32_t jla = (uint32_t)__shfl((int32_t)la, j), jlb = (uint32_t)__shfl((int32_t)lb, j);
+            const bool a_short = ja <= jb;
+            const uint32_t ls = a_short ? jla : jlb, ll = a_short ? jlb : jla;
+            const int32_t ns = min(ja, jb), nl = max(ja, jb);
+            for (int32_t i0 = 0; i0 < ns; i0 += WAVE) {
+                const int32_t i = i0 + lane;
+                if (i < ns) {
+                    const int32_t x = E[ls + i];
+                    int32_t lo = 0, len = nl;
+                    while (len > 0) {
+                        const int32_t half = len >> 1;
+                        const bool right = E[ll + lo + half] < x;
+                        lo = right ? lo + half + 1 : lo;
+                        len = right ? len - half - 1 : half;
+                    }
+                    if (lo < nl && E[ll + lo] == x)
+                        acc -= (uint64_t)(uint32_t)(SUP[ls + i] - 1) * (uint64_t)(uint32_t)(SUP[ll + lo] - 1);
+                }
+            }
+        }
+    }
+    const uint64_t tot = wave_sum_u64(acc);
+    if (lane == 0 && tot) atomicAdd(&out[0], (unsigned long long)tot);
+}
+
 } // namespace
+
+hipError_t gp_vertex_triangles_launch(const uint32_t* vertex, const int32_t* sup, int32_t v_cnt, uint64_t* t,
+                                      hipStream_t stream) {
+    const unsigned blocks = (unsigned)std::min<int64_t>(4096, ((int64_t)v_cnt + 255) / 256 + 1);
+    hipLaunchKernelGGL(vertex_triangles_kernel, dim3(blocks), dim3(256), 0, stream, vertex, sup, v_cnt, t);
+    return hipGetLastError();
+}
+
+hipError_t gp_tri_edge_launch(const uint32_t* vertex, const int32_t* edge, const int32_t* src, const int32_t* sup,
+                              const uint64_t* t, uint64_t e_cnt, unsigned long long* counter, unsigned long long* out,
+                              unsigned waves, hipStream_t stream) {
+    hipLaunchKernelGGL(tri_edge_kernel, dim3(waves), dim3(WAVE), 0, stream, vertex, edge, src, sup, t, e_cnt, counter,
+                       out);
+    return hipGetLastError();
+}
 
 hipError_t gp_house_launch(const GpHouseArgs& args, unsigned waves, hipStream_t stream) {
     void* kargs[] = {const_cast<GpHouseArgs*>(&args)};

@@ -330,7 +330,79 @@ extern "C" int gp_kernel_plan(const gp_sched_flat* sched, gp_plan_info* out) {
 // is depth d adds N(v_d)); the loop set of depth d must be the intersection
 // over d's earlier neighbours in the schedule-ordered adjacency, as the
 // planner builds it (src/schedule.cpp:329-362).
+namespace {
+
+// the schedule's loop sets and IEP groups are the ones `want` (row-major
+// n x n, schedule order) describes, with a two-vertex IEP leaf
+bool loop_nest_is(const gp_sched_flat* s, int n, const int* want) {
+    if (s->size != n || s->iep_num != 2 || s->iep_group_num != 2) return false;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if ((s->adj[i * n + j] != 0) != (want[i * n + j] != 0)) return false;
+    // prefix p -> depth mask of the neighbourhoods it intersects
+    const int P = s->total_prefix_num;
+    if (P < 1 || P > GP_MAX_PREFIX) return false;
+    int depth_of[GP_MAX_PREFIX];
+    for (int p = 0; p < P; ++p) depth_of[p] = -1;
+    for (int d = 0; d < n; ++d)
+        for (int p = s->last[d], guard = 0; p != -1 && guard <= P; p = s->next[p], ++guard) {
+            if (p < 0 || p >= P) return false;
+            depth_of[p] = d;
+        }
+    uint32_t mask[GP_MAX_PREFIX];
+    for (int p = 0; p < P; ++p) {
+        uint32_t m = 0;
+        for (int q = p, guard = 0; q != -1 && guard <= P; q = s->father[q], ++guard) {
+            if (q < 0 || q >= P || depth_of[q] < 0) return false;
+            m |= 1u << depth_of[q];
+        }
+        mask[p] = m;
+    }
+    for (int d = 1; d < n; ++d) {
+        const int q = s->loop_set[d];
+        if (q < 0 || q >= P) return false;
+        uint32_t w = 0;
+        for (int j = 0; j < d; ++j)
+            if (want[d * n + j]) w |= 1u << j;
+        if (mask[q] != w) return false;
+    }
+    // |A||B| - |A ∩ B| (two singleton blocks with weight +1, one block of
+    // both with weight -1), in either group order
+    int singles = -1, joint = -1;
+    for (int g = 0; g < 2; ++g) {
+        const int b0 = s->iep_block[g][0], b1 = s->iep_block[g][1];
+        if (b0 != b1 && s->iep_val[g] == 1) singles = g;
+        if (b0 == b1 && s->iep_val[g] == -1) joint = g;
+    }
+    return singles >= 0 && joint >= 0;
+}
+
+// the restrictions the loops check (depths below the IEP leaf): exactly
+// `want_index` at depth `want_depth` (or none when want_depth < 0)
+bool loop_restrictions_are(const gp_sched_flat* s, int leaf, int want_depth, int want_index) {
+    for (int d = 0; d < leaf; ++d) {
+        const int r = s->restrict_last[d];
+        if (d == want_depth) {
+            if (r < 0 || r >= GP_MAX_RESTRICT || s->restrict_index[r] != want_index || s->restrict_next[r] != -1)
+                return false;
+        } else if (r != -1) {
+            return false;
+        }
+    }
+    return true;
+}
+
+} // namespace
+
 int gp_sched_shape(const gp_sched_flat* s) {
+    // two triangles joined by an edge: triangle (v0, v1, v5), bridge v1-v2,
+    // triangle (v2, v3, v4); loops v0, v1, v2, v3 with v3 < v0; the
+    // restrictions on v4 and v5 sit at IEP depths, which the loop nest does
+    // not check, and the redundancy 4 makes up for them
+    static const int tet[36] = {0, 1, 0, 0, 0, 1, 1, 0, 1, 0, 0, 1, 0, 1, 0, 1, 1, 0,
+                                0, 0, 1, 0, 1, 0, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0, 0, 0};
+    if (loop_nest_is(s, 6, tet) && loop_restrictions_are(s, 4, 3, 0) && s->redundancy == 4)
+        return GP_SHAPE_TRI_EDGE_TRI;
     const int n = s->size;
     if (n != 5 || s->iep_num != 2 || s->iep_group_num != 2) return GP_SHAPE_GENERIC;
     static const int house[5][5] = {{0, 1, 1, 1, 0}, {1, 0, 0, 1, 1}, {1, 0, 0, 0, 1}, {1, 1, 0, 0, 0}, {0, 1, 1, 0, 0}};

@@ -566,3 +566,45 @@ def test_house_twohop_kernel_matches(graph, rmat13_path, wiki_vote_path):
         gold = [r["count"] for r in load_json("reference_counts.json")
                 if r["graph"] == "wiki-vote" and r["pattern"] == "house" and (r["pm"], r["rt"], r["iep"]) == (1, 1, 1)]
         assert dev.pattern_matching(_sched(host, gp.PATTERNS["house"], 1, 1, 1, tri)) == gold[0]
+
+
+CFG3_STATS = (2999998, 234478088, 3556476596)  # the 3M-vertex Chung-Lu graph's v_cnt, e_cnt, triangles
+
+
+@pytest.mark.parametrize("graph", ["rand1", "rand2", "hubs", "wiki"])
+def test_tri_edge_tri_closed_form_matches(graph, wiki_vote_path):
+    """Two triangles joined by an edge under the GraphPi schedule (the one the
+    planner builds for the 3M-vertex Chung-Lu graph's statistics, and for
+    wiki-vote's) is counted in closed form from triangles per vertex, edge
+    supports and the 4-clique count.  It must equal the oracle's loop nest,
+    raw sum included (4 x the count), the generic kernel (no_special), and
+    the generic kernel's task ranges added up."""
+    if graph == "rand1":
+        host = _random_graph(60, 0.2, 31)
+    elif graph == "rand2":
+        host = _random_graph(150, 0.06, 32, hubs=3)
+    elif graph == "hubs":
+        host = gp.HostGraph.chung_lu(2000, 16000, 2.1, 200.0, 3)
+    else:
+        host = gp.HostGraph.load_text(wiki_vote_path)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    stats = CFG3_STATS if graph != "wiki" else (host.v_cnt, host.e_cnt, tri)
+    s, valid = gp.build_schedule(gp.PATTERNS["tri_edge_tri"], 1, 1, True, *stats)
+    assert valid and gp.kernel_plan(s)["shape"] == 3 and s.redundancy == 4
+    got = dev.count(s)
+    assert got.stats.kernel_variant == 3
+    gen = dev.count(s, _generic())
+    print(f"{graph} tri_edge_tri: {got.count} closed form {got.stats.kernel_ms:.2f} ms, generic "
+          f"{gen.stats.kernel_ms:.2f} ms", flush=True)
+    assert (got.raw, got.count) == (gen.raw, gen.count)
+    assert got.raw == (4 * got.count) % (1 << 64)
+    if graph == "wiki":
+        gold = [r["count"] for r in load_json("reference_counts.json")
+                if r["graph"] == "wiki-vote" and r["pattern"] == "tri_edge_tri"]
+        assert got.count == gold[0]
+    else:
+        want = ob.pattern_matching(host, s, threads=8)
+        assert (got.raw, got.count) == (want["raw"], want["count"])
+    # task ranges take the generic kernel and still add up to the same raw sum
+    assert dev.count_sliced(s, 5).raw == got.raw

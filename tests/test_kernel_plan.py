@@ -68,8 +68,10 @@ def test_specialised_shapes():
     assert shape("house", 1, 0) == 2
     assert shape("house", 1, 1, False) == 0   # no IEP: a plain leaf at depth 4
     assert shape("house", 2, 1) == 0          # the GraphZero model picks another vertex order
+    assert shape("tri_edge_tri") == 3          # closed form from triangles, supports and 4-cliques
+    assert shape("tri_edge_tri", 1, 1, False) == 0
     for name in sorted(EXPECTED):
-        if name != "house":
+        if name not in ("house", "tri_edge_tri"):
             assert shape(name) == 0, name
     # a tampered restriction (v2 < v0 added at depth 2) is not the House shape
     s, _ = gp.build_schedule(gp.PATTERNS["house"], 1, 1, True, *STATS)
