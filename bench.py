@@ -353,8 +353,8 @@ def main():
         """One full count of `name` as consecutive task ranges (progress is
         logged per range): a first range of 1/256 of the tasks (1/4 for a
         kernel whose unit is a root); the rest in one
-        launch when that is predicted to take under 10 s even at 4 times the
-        first range's rate; otherwise ranges at most twice the previous one and
+        launch when that is predicted to take under target_ms even at 4 times
+        the first range's rate; otherwise ranges at most twice the previous one and
         sized for about target_ms at the previous range's rate (task cost
         varies along the task order, so each launch stays bounded).  The raw
         sums of the ranges add up to the whole count."""
@@ -363,6 +363,15 @@ def main():
         tun = gp.Tuning()
         if tuning is not None:
             ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
+        if gp.kernel_plan(s)["shape"] == 3 and not tun.collect_balg:
+            # counted in closed form (triangles, edge supports, 4-cliques):
+            # one call for the whole graph, a task range would take the loop nest
+            t0 = time.perf_counter()
+            r = dev.count(s, tun)
+            if not quiet:
+                log(f"    {name}: closed form, {r.stats.kernel_ms:.1f} ms in {r.stats.launches} launches")
+            return {"count": r.count, "raw": r.raw, "kernel_ms": r.stats.kernel_ms, "launches": r.stats.launches,
+                    "wall_s": time.perf_counter() - t0, "balg": 0}
         tun.slice_ms = -1  # the ranges below are the launches
         # a kernel that takes whole roots (House's two-hop kernel) gets a
         # first range of 1/4: a short first range holds only a few heavy
@@ -387,7 +396,7 @@ def main():
                     f"all {r.stats.finish_max_ms:.1f} ms; {r.stats.split_tasks} tasks split)")
             rate = max(k, 0.01) / max(1, done - begin)   # ms per task in the last range
             size = max(1, min(2 * (done - begin), int(target_ms / rate)))
-            if total - done <= size + size // 4 or (launches == 1 and 4 * rate * (total - done) < 10000.0):
+            if total - done <= size + size // 4 or (launches == 1 and 4 * rate * (total - done) < target_ms):
                 size = total - done
             begin = done
         signed = raw - (1 << 64) if raw >= (1 << 63) else raw
