@@ -139,7 +139,18 @@ class Stats(ctypes.Structure):
         ("launches", ctypes.c_int32),
         ("finished", ctypes.c_int32),
         ("tasks_total", ctypes.c_uint64),
+        ("exact_count_valid", ctypes.c_int32),
+        ("exact_count_lo", ctypes.c_uint64),
+        ("exact_count_hi", ctypes.c_uint64),
     ]
+
+    def exact_count(self):
+        """The count without wrap-around (signed 128-bit) when a closed form
+        computed it, else None."""
+        if not self.exact_count_valid:
+            return None
+        v = (self.exact_count_hi << 64) | self.exact_count_lo
+        return v - (1 << 128) if v >> 127 else v
 
 
 class PlanInfo(ctypes.Structure):

@@ -2499,7 +2499,13 @@ int run_tri_edge_tri(gp_graph* g, const gp_sched_flat* sched, uint64_t* raw, int
     CallCtx* c = nullptr;
     rc = acquire_ctx(g, 64, &c);
     if (rc != GP_OK) return rc;
+    const unsigned waves = (unsigned)(g->num_cu * 16);
     hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess && c->clocks_cap < (size_t)waves) { // the per-wave 128-bit sums go there
+        size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
+        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves);
+        c->clocks_cap = e == hipSuccess ? (size_t)waves : 0;
+    }
     if (e != hipSuccess) {
         destroy_ctx(c);
         return gp_fail(GP_ENOMEM, std::string("call buffers: ") + hipGetErrorString(e));
@@ -2530,14 +2536,18 @@ int run_tri_edge_tri(gp_graph* g, const gp_sched_flat* sched, uint64_t* raw, int
         }
     }
     unsigned long long host_out[8] = {0};
+    std::vector<unsigned long long> wide(2 * (size_t)waves, 0);
     hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, 8);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(c->e0, c->stream);
     if (e == hipSuccess && g->e_cnt)
         e = gp_tri_edge_launch(g->d_vertex, g->d_edge, g->d_src, g->d_sup, g->d_vtri, g->e_cnt, c->d_cnt + 1,
-                               c->d_cnt + 3, (unsigned)(g->num_cu * 16), c->stream);
+                               c->d_cnt + 3, c->d_clocks, waves, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host_out, c->d_cnt, sizeof(host_out), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && g->e_cnt)
+        e = hipMemcpyAsync(wide.data(), c->d_clocks, sizeof(unsigned long long) * wide.size(), hipMemcpyDeviceToHost,
+                           c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms_e, c->e0, c->e1);
     if (e != hipSuccess) {
@@ -2549,8 +2559,16 @@ int run_tri_edge_tri(gp_graph* g, const gp_sched_flat* sched, uint64_t* raw, int
     const uint64_t r = cnt * (uint64_t)sched->redundancy; // the loop nest's raw sum, 4 C
     if (raw) *raw = r;
     if (count) *count = (int64_t)r / sched->redundancy;
+    // the count itself in 128 bits (the reference's long long wraps past 2^63)
+    __int128 exact = (__int128)k4 * 6;
+    if (g->e_cnt)
+        for (unsigned w = 0; w < waves; ++w)
+            exact += (__int128)(((unsigned __int128)wide[2 * w + 1] << 64) | wide[2 * w]);
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
+        stats->exact_count_valid = 1;
+        stats->exact_count_lo = (uint64_t)(unsigned __int128)exact;
+        stats->exact_count_hi = (uint64_t)((unsigned __int128)exact >> 64);
         stats->kernel_ms = st4.kernel_ms + ms_t + ms_e + (sup_now ? g->sup_build_ms : 0.0);
         stats->tasks = g->e_cnt;
         stats->tasks_total = g->e_cnt;

@@ -30,7 +30,8 @@ hipError_t gp_support_launch(const uint32_t* vertex, const int32_t* edge, const 
 hipError_t gp_vertex_triangles_launch(const uint32_t* vertex, const int32_t* sup, int32_t v_cnt, uint64_t* t,
                                       hipStream_t stream);
 // out[0] += sum over edges {a, b} (a > b) of (t_a - s)(t_b - s) - sum_{v in N(a) ∩ N(b)} (s_av - 1)(s_bv - 1),
-// s = sup(a, b), all in u64 (two's-complement wrap-around)
+// s = sup(a, b), in u64 (two's-complement wrap-around); out128[2w], out128[2w+1] = wave w's share of the
+// same sum in 128 bits (low, high word; signed)
 hipError_t gp_tri_edge_launch(const uint32_t* vertex, const int32_t* edge, const int32_t* src, const int32_t* sup,
                               const uint64_t* t, uint64_t e_cnt, unsigned long long* counter, unsigned long long* out,
-                              unsigned waves, hipStream_t stream);
+                              unsigned long long* out128, unsigned waves, hipStream_t stream);

@@ -353,11 +353,14 @@ __global__ void __launch_bounds__(WAVE) tri_edge_kernel(const uint32_t* __restri
                                                         const int32_t* __restrict__ src,
                                                         const int32_t* __restrict__ sup,
                                                         const uint64_t* __restrict__ t, uint64_t e_cnt,
-                                                        unsigned long long* counter, unsigned long long* out) {
+                                                        unsigned long long* counter, unsigned long long* out,
+                                                        unsigned long long* out128) {
+    __shared__ unsigned long long s_lo[WAVE], s_hi[WAVE];
     const int lane = lane_id();
     const gcptr E = (gcptr)edge;
     const gcptr SUP = (gcptr)sup;
-    uint64_t acc = 0; // per lane, u64 wrap-around
+    uint64_t acc = 0;    // per lane, u64 wrap-around
+    __int128 exact = 0;  // per lane, the same sum in 128 bits (no wrap-around)
     for (;;) {
         unsigned long long base = 0;
         if (lane == 0) base = atomicAdd(counter, (unsigned long long)WAVE);
@@ -378,7 +381,9 @@ __global__ void __launch_bounds__(WAVE) tri_edge_kernel(const uint32_t* __restri
                 lb = vertex[b];
                 db = (int32_t)(vertex[b + 1] - lb);
                 const uint64_t s = (uint32_t)SUP[e];
-                acc += (t[a] - s) * (t[b] - s);
+                const uint64_t p = (t[a] - s) * (t[b] - s); // each factor < 2^32: exact in 64 bits
+                acc += p;
+                exact += (__int128)p;
             }
         }
         const bool small = mine && min(da, db) <= 8;
@@ -395,8 +400,11 @@ __global__ void __launch_bounds__(WAVE) tri_edge_kernel(const uint32_t* __restri
                     lo = right ? lo + half + 1 : lo;
                     len = right ? len - half - 1 : half;
                 }
-                if (lo < nl && E[ll + lo] == x)
-                    acc -= (uint64_t)(uint32_t)(SUP[ls + i] - 1) * (uint64_t)(uint32_t)(SUP[ll + lo] - 1);
+                if (lo < nl && E[ll + lo] == x) {
+                    const uint64_t q = (uint64_t)(uint32_t)(SUP[ls + i] - 1) * (uint64_t)(uint32_t)(SUP[ll + lo] - 1);
+                    acc -= q;
+                    exact -= (__int128)q;
+                }
             }
         }
         uint64_t big = __ballot(mine && !small);
@@ -419,14 +427,28 @@ __global__ void __launch_bounds__(WAVE) tri_edge_kernel(const uint32_t* __restri
                         lo = right ? lo + half + 1 : lo;
                         len = right ? len - half - 1 : half;
                     }
-                    if (lo < nl && E[ll + lo] == x)
-                        acc -= (uint64_t)(uint32_t)(SUP[ls + i] - 1) * (uint64_t)(uint32_t)(SUP[ll + lo] - 1);
+                    if (lo < nl && E[ll + lo] == x) {
+                        const uint64_t q =
+                            (uint64_t)(uint32_t)(SUP[ls + i] - 1) * (uint64_t)(uint32_t)(SUP[ll + lo] - 1);
+                        acc -= q;
+                        exact -= (__int128)q;
+                    }
                 }
             }
         }
     }
     const uint64_t tot = wave_sum_u64(acc);
     if (lane == 0 && tot) atomicAdd(&out[0], (unsigned long long)tot);
+    // the 128-bit sum: lane 0 adds the lanes' values, one slot per wave
+    s_lo[lane] = (unsigned long long)(unsigned __int128)exact;
+    s_hi[lane] = (unsigned long long)((unsigned __int128)exact >> 64);
+    __syncthreads();
+    if (lane == 0) {
+        unsigned __int128 w = 0;
+        for (int i = 0; i < WAVE; ++i) w += ((unsigned __int128)s_hi[i] << 64) | s_lo[i];
+        out128[2 * blockIdx.x] = (unsigned long long)w;
+        out128[2 * blockIdx.x + 1] = (unsigned long long)(w >> 64);
+    }
 }
 
 } // namespace
@@ -440,9 +462,9 @@ hipError_t gp_vertex_triangles_launch(const uint32_t* vertex, const int32_t* sup
 
 hipError_t gp_tri_edge_launch(const uint32_t* vertex, const int32_t* edge, const int32_t* src, const int32_t* sup,
                               const uint64_t* t, uint64_t e_cnt, unsigned long long* counter, unsigned long long* out,
-                              unsigned waves, hipStream_t stream) {
+                              unsigned long long* out128, unsigned waves, hipStream_t stream) {
     hipLaunchKernelGGL(tri_edge_kernel, dim3(waves), dim3(WAVE), 0, stream, vertex, edge, src, sup, t, e_cnt, counter,
-                       out);
+                       out, out128);
     return hipGetLastError();
 }
 

@@ -153,6 +153,12 @@ typedef struct gp_stats {
     int32_t launches;       /* count launches (task ranges) the call made */
     int32_t finished;       /* 1 when every task was counted; 0 after GP_ETIME */
     uint64_t tasks_total;   /* tasks the call was asked to count */
+    int32_t exact_count_valid; /* 1 when the count is also known without wrap-around (a closed form
+                                  computed it in 128 bits): exact_count_hi:lo, a signed 128-bit
+                                  integer.  The returned count follows the reference's long long
+                                  arithmetic, which wraps past 2^63 */
+    uint64_t exact_count_lo;
+    uint64_t exact_count_hi;
 } gp_stats;
 
 typedef struct gp_graph gp_graph; /* device-resident CSR (opaque) */

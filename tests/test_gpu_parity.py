@@ -599,6 +599,7 @@ def test_tri_edge_tri_closed_form_matches(graph, wiki_vote_path):
           f"{gen.stats.kernel_ms:.2f} ms", flush=True)
     assert (got.raw, got.count) == (gen.raw, gen.count)
     assert got.raw == (4 * got.count) % (1 << 64)
+    assert got.stats.exact_count() == got.count  # no wrap-around at these sizes
     if graph == "wiki":
         gold = [r["count"] for r in load_json("reference_counts.json")
                 if r["graph"] == "wiki-vote" and r["pattern"] == "tri_edge_tri"]
