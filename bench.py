@@ -110,17 +110,20 @@ def cpu_baseline(host, scheds, names, budget_s=15.0, seed=7):
         done = r["roots_done"]
         if done == host.v_cnt:
             est, kind = wall, "measured"
+        elif done == 0:
+            # not one root finished: there is nothing to scale
+            est, kind = None, f"unmeasured: no root finished within {budget_s:g} s"
         else:
             est = r["busy_s"] / max(1, done) * host.v_cnt / threads
             kind = "lower-bound estimate"
-        out[name] = {"value": round(est, 3), "kind": kind, "roots_done": done, "roots": host.v_cnt,
-                     "sample_s": round(wall, 2)}
+        out[name] = {"value": None if est is None else round(est, 3), "kind": kind, "roots_done": done,
+                     "roots": host.v_cnt, "sample_s": round(wall, 2)}
         log(f"cpu {name}: {done} of {host.v_cnt} random roots in {wall:.1f} s on {threads} threads -> "
-            f"{est:.1f} s ({kind})")
+            f"{'-' if est is None else f'{est:.1f}'} s ({kind})")
     return cpu, out
 
 
-def batched_bench(gp, scale=10):
+def batched_bench(gp, scale=12):
     """Every connected 5- and 6-vertex pattern (21 + 112, MotifGenerator order)
     over one resident R-MAT graph (edge factor 16, seed 4): one gp_count_batch
     call against the same schedules counted one gp_count call at a time."""
@@ -168,52 +171,106 @@ def batched_bench(gp, scale=10):
             "speedup": round(loop_wall / batch_wall, 3), "exact": exact}
 
 
-def served_bench(gp, scales=(8, 9, 10, 11), rates=(50, 200, 800), n_req=120, seed=12, use_graph=0):
-    """Counting requests against resident graphs (R-MAT edge factor 16, seeds
-    5.., one per scale), random connected 3-6-vertex patterns (spanning tree +
-    Bernoulli(0.3) extra edges), Poisson arrivals at each rate: latency p50/p99
-    and throughput of the serving loop, against the same requests served one at
-    a time by plain gp_count calls (a serial queue).  use_graph = 1 runs the
-    loop's launches as hipGraphs (default: stream launches)."""
+def served_bench(gp, scales=(16, 20, 22, 24), rates=(50, 200, 800), n_req=60, seed=12, deadline_s=1.0,
+                 quantum_s=0.1, use_graph=0):
+    """Counting requests against resident graphs (R-MAT edge factor 16,
+    seeds 5.., one per scale), random connected 3-7-vertex patterns (uniform
+    size; spanning tree + Bernoulli(0.3) extra edges) on a uniformly chosen
+    graph, Poisson arrivals at each rate; every request has a deadline of
+    deadline_s after its arrival.
+
+    The loop: CountServer (4 slots, batches of up to 32, time slices of
+    quantum_s, admission by footprint within 85 % of the free HBM).  Serial:
+    the same requests one at a time through plain gp_count with the same
+    deadline as its time limit, first come first served at the same
+    arrivals.  Each request's service time alone is measured once; the
+    serial latencies follow from those (a request whose deadline has passed
+    when its turn comes, or would pass during its count, times out).  p50 and
+    p99 are over answered requests; timed-out requests are reported by class
+    (pattern size, graph), never dropped.  Throughput counts answered
+    requests.  Every answered result equals the count made alone."""
     import numpy as np
     from graphpi_amd.serve import CountServer, Request, percentile, poisson_arrivals, random_pattern, run_load
-    graphs = {f"s{sc}": gp.HostGraph.rmat(sc, 16, 0.57, 0.19, 0.19, 5 + i) for i, sc in enumerate(scales)}
-    names = sorted(graphs)
+    t0 = time.perf_counter()
+    graphs = {}
+    for i, sc in enumerate(scales):
+        graphs[f"s{sc}"] = gp.HostGraph.rmat(sc, 16, 0.57, 0.19, 0.19, 5 + i)
+        log(f"served: R-MAT s{sc} ef16 seed {5 + i}: {graphs[f's{sc}'].v_cnt} vertices, "
+            f"{graphs[f's{sc}'].e_cnt // 2} edges ({time.perf_counter() - t0:.1f} s)")
+    names = sorted(graphs, key=lambda x: int(x[1:]))
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n_req):
+        g = names[int(rng.integers(0, len(names)))]
+        specs.append((g, random_pattern(rng, int(rng.integers(3, 8)))))
     out = {"workload": "R-MAT scales " + "/".join(str(x) for x in scales) +
-                       ", edge factor 16, seeds 5-8; random connected 3-6-vertex patterns; Poisson arrivals",
-           "requests_per_rate": n_req, "rates": {}}
+                       f", edge factor 16, seeds 5-{4 + len(scales)}; random connected 3-7-vertex patterns; "
+                       f"Poisson arrivals; deadline {deadline_s:g} s after arrival",
+           "requests_per_rate": n_req, "deadline_s": deadline_s, "quantum_s": quantum_s, "rates": {}}
     tun = gp.Tuning()
     tun.use_graph = use_graph
-    with CountServer(graphs, slots=4, max_batch=32, tuning=tun) as srv:
-        alone = {nm: srv._dev[nm] for nm in names}
+    with CountServer(graphs, slots=4, max_batch=32, tuning=tun, quantum_s=quantum_s, deadline_s=deadline_s) as srv:
+        out["resident_graph_bytes"] = sum(d.nbytes() for d in srv._dev.values())
+        out["admission_budget_bytes"] = srv._budget
+        scheds = [srv.schedule_for(g, adj, (1, 1, 1)) for g, adj in specs]
+        # each request alone, once: plain gp_count with the deadline as its time limit
+        service, alone = [], []
+        for (g, adj), s in zip(specs, scheds):
+            t = gp.Tuning()
+            t.time_limit_s = deadline_s
+            t.slice_ms = max(1, int(quantum_s * 1000))
+            a = time.perf_counter()
+            try:
+                r = srv._dev[g].count(s, t)
+            except gp.GPTimeout:
+                r = None
+            service.append(time.perf_counter() - a)
+            alone.append(r)
+        log(f"served: {sum(r is not None for r in alone)} of {n_req} requests finish alone within "
+            f"{deadline_s:g} s ({sum(service):.1f} s of service in all)")
+        classes = {}
+        for (g, adj), r in zip(specs, alone):
+            c = classes.setdefault(f"{adj.shape[0]}v {g}", {"requests": 0, "alone_finished": 0})
+            c["requests"] += 1
+            c["alone_finished"] += r is not None
         for rate in rates:
-            rng = np.random.default_rng(seed + rate)
-            reqs = [Request(i, names[int(rng.integers(0, len(names)))], random_pattern(rng, int(rng.integers(3, 7))))
-                    for i in range(n_req)]
-            offs = poisson_arrivals(rng, float(rate), n_req)
-            for r in reqs:  # plan outside the timed region, as a warm cache would
-                srv.schedule_for(r.graph, r.adj, r.settings)
+            rrng = np.random.default_rng(seed + rate)
+            reqs = [Request(i, g, adj) for i, (g, adj) in enumerate(specs)]
+            offs = poisson_arrivals(rrng, float(rate), n_req)
             wall = run_load(srv, reqs, offs)
-            lat = [r.latency * 1e3 for r in reqs]
-            # the same requests, each alone, queued serially at the same arrivals
-            service, exact = [], True
+            ans = [r for r in reqs if r.error is None]
+            exact = all(a is None or (r.count, r.raw) == (a.count, a.raw) for r, a in zip(ans, [alone[r.rid] for r in ans]))
+            lat = [r.latency * 1e3 for r in ans]
+            # the serial queue at the same arrivals
+            clock, slat, s_ok = 0.0, [], 0
+            for off, sv, a in zip(offs, service, alone):
+                start = max(off, clock)
+                budget = off + deadline_s - start
+                if budget <= 0:
+                    clock = start
+                elif a is not None and sv <= budget:
+                    clock = start + sv
+                    slat.append((clock - off) * 1e3)
+                    s_ok += 1
+                else:
+                    clock = start + min(sv, budget)
+            per_class = {}
             for r in reqs:
-                t0 = time.perf_counter()
-                one = alone[r.graph].count(srv.schedule_for(r.graph, r.adj, r.settings))
-                service.append(time.perf_counter() - t0)
-                exact = exact and (one.count, one.raw) == (r.count, r.raw)
-            done, slat = 0.0, []
-            for off, sv in zip(offs, service):
-                done = max(done, off) + sv
-                slat.append((done - off) * 1e3)
-            serial_wall = done
-            rec = {"p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
-                   "throughput_qps": round(n_req / wall, 2),
-                   "serial_p50_ms": round(percentile(slat, 50), 3), "serial_p99_ms": round(percentile(slat, 99), 3),
-                   "serial_throughput_qps": round(n_req / serial_wall, 2), "exact": exact}
-            log(f"served {rate} q/s: {rec}")
+                c = per_class.setdefault(f"{r.adj.shape[0]}v {r.graph}", {"answered": 0, "timed_out": 0})
+                c["answered" if r.error is None else "timed_out"] += 1
+            rec = {"answered": len(ans), "timed_out": sum(r.timed_out for r in reqs),
+                   "p50_ms": round(percentile(lat, 50), 3) if lat else None,
+                   "p99_ms": round(percentile(lat, 99), 3) if lat else None,
+                   "throughput_qps": round(len(ans) / wall, 2),
+                   "serial_answered": s_ok,
+                   "serial_p50_ms": round(percentile(slat, 50), 3) if slat else None,
+                   "serial_p99_ms": round(percentile(slat, 99), 3) if slat else None,
+                   "serial_throughput_qps": round(s_ok / clock, 2) if clock > 0 else None,
+                   "slices_max": max(r.slices for r in reqs), "exact": exact, "classes": per_class}
+            log(f"served {rate} q/s: " + json.dumps({k: v for k, v in rec.items() if k != "classes"}))
             assert exact
             out["rates"][str(rate)] = rec
+        out["classes_alone"] = classes
     return out
 
 
@@ -240,10 +297,13 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--autotune", action="store_true",
                     help="pick launch settings per pattern on a task sample before the timed region")
-    ap.add_argument("--batched-scale", type=int, default=10,
-                    help="R-MAT scale of the many-pattern batched measurement (default 10)")
-    ap.add_argument("--served-scales", default="8,9,10,11",
-                    help="R-MAT scales of the served graphs (default 8,9,10,11)")
+    ap.add_argument("--batched-scale", type=int, default=12,
+                    help="R-MAT scale of the many-pattern batched measurement (default 12)")
+    ap.add_argument("--served-scales", default="16,20,22,24",
+                    help="R-MAT scales of the served graphs (default 16,20,22,24)")
+    ap.add_argument("--served-requests", type=int, default=60, help="requests per arrival rate (default 60)")
+    ap.add_argument("--served-deadline", type=float, default=1.0,
+                    help="seconds after arrival by which a served request must be answered (default 1)")
     ap.add_argument("--patterns", default=None, help="comma-separated timed patterns (exploration only)")
     ap.add_argument("--once", default=None,
                     help="comma-separated patterns counted once outside the steps ('' for none)")
@@ -351,54 +411,61 @@ def main():
             tunings[name], tune_reports[name] = autotune(dev, scheds[name], log=log)
 
     def count_pattern(name, target_ms=5000.0, tuning=None, quiet=False):
-        """One full count of `name` as consecutive task ranges (progress is
-        logged per range): a first range of 1/256 of the tasks (1/4 for a
-        kernel whose unit is a root); the rest in one
-        launch when that is predicted to take under target_ms even at 4 times
-        the first range's rate; otherwise ranges at most twice the previous one and
-        sized for about target_ms at the previous range's rate (task cost
-        varies along the task order, so each launch stays bounded).  The raw
-        sums of the ranges add up to the whole count."""
+        """One full count of `name` as consecutive launches, with a progress
+        line per launch.  The generic kernel runs with a deadline of
+        target_ms per launch: each gp_count call makes one launch, which
+        stops fetching at the deadline and counts a prefix of the remaining
+        tasks (the call's time limit then returns it), and the next call
+        resumes there.  House's two-hop kernel takes a root per fetch and no
+        deadline: a first range of 1/4 of its tasks, then ranges at most four
+        times the previous one and sized for about target_ms at the previous
+        range's rate.  The raw sums add up to the whole count."""
         s = scheds[name]
         total = totals[name]
         tun = gp.Tuning()
         if tuning is not None:
             ctypes.memmove(ctypes.byref(tun), ctypes.byref(tuning), ctypes.sizeof(gp.Tuning))
-        if gp.kernel_plan(s)["shape"] == 3 and not tun.collect_balg:
+        shape = 0 if (tun.collect_balg or tun.no_special) else gp.kernel_plan(s)["shape"]
+        t0 = time.perf_counter()
+        if shape == 3:
             # counted in closed form (triangles, edge supports, 4-cliques):
             # one call for the whole graph, a task range would take the loop nest
-            t0 = time.perf_counter()
             r = dev.count(s, tun)
             if not quiet:
                 log(f"    {name}: closed form, {r.stats.kernel_ms:.1f} ms in {r.stats.launches} launches")
             return {"count": r.count, "raw": r.raw, "kernel_ms": r.stats.kernel_ms, "launches": r.stats.launches,
                     "wall_s": time.perf_counter() - t0, "balg": 0, "exact": r.stats.exact_count()}
-        tun.slice_ms = -1  # the ranges below are the launches
-        # a kernel that takes whole roots (House's two-hop kernel) gets a
-        # first range of 1/4: a short first range holds only a few heavy
-        # roots and leaves most waves idle
-        div = 4 if gp.kernel_plan(s)["shape"] else 256
-        first = total if total <= (1 << 20) else max(1, total // div)
-        begin, size, raw, ms, launches = 0, first, 0, 0.0, 0
-        balg_sum = 0
-        t0 = time.perf_counter()
+        begin, raw, ms, launches, balg_sum = 0, 0, 0.0, 0, 0
+        size = total if total <= (1 << 20) else max(1, total // 4)
         while begin < total:
             tun.task_begin = begin
-            tun.task_end = min(total, begin + size)
-            r = dev.count(s, tun)
+            if shape:
+                tun.task_end = min(total, begin + size)
+                tun.slice_ms = -1
+                r = dev.count(s, tun)
+                done = tun.task_end
+            else:
+                tun.task_end = total
+                tun.slice_ms = int(target_ms)
+                tun.time_limit_s = 1e-9  # one launch per call
+                try:
+                    r = dev.count(s, tun)
+                except gp.GPTimeout as e:
+                    r = e.partial
+                done = begin + r.stats.tasks
             raw = (raw + r.raw) % (1 << 64)
             ms += r.stats.kernel_ms
             balg_sum += r.stats.balg_bytes
-            launches += 1
-            done = tun.task_end
+            launches += r.stats.launches
             k = r.stats.kernel_ms
             if not quiet:
                 log(f"    {name}: tasks {begin}..{done} of {total}: {k:.1f} ms (tail: half {r.stats.finish_p50_ms:.1f}, "
                     f"all {r.stats.finish_max_ms:.1f} ms; {r.stats.split_tasks} tasks split)")
-            rate = max(k, 0.01) / max(1, done - begin)   # ms per task in the last range
-            size = max(1, min(2 * (done - begin), int(target_ms / rate)))
-            if total - done <= size + size // 4 or (launches == 1 and 4 * rate * (total - done) < target_ms):
-                size = total - done
+            if shape:
+                rate = max(k, 0.01) / max(1, done - begin)   # ms per task in the last range
+                size = max(1, min(4 * (done - begin), int(target_ms / rate)))
+                if total - done <= size + size // 4:
+                    size = total - done
             begin = done
         signed = raw - (1 << 64) if raw >= (1 << 63) else raw
         red = s.redundancy or 1
@@ -527,7 +594,8 @@ def main():
 
     batched = None if args.no_batched else batched_bench(gp, scale=args.batched_scale)
     served = None if args.no_served else served_bench(
-        gp, scales=tuple(int(x) for x in args.served_scales.split(",")))
+        gp, scales=tuple(int(x) for x in args.served_scales.split(",")), n_req=args.served_requests,
+        deadline_s=args.served_deadline)
 
     cpu = None
     if not args.no_cpu:
@@ -535,8 +603,9 @@ def main():
         for n, c in per.items():
             per_pattern[n]["cpu_s"] = c["value"]
             per_pattern[n]["cpu_kind"] = c["kind"]
-            per_pattern[n]["gpu_speedup"] = round(c["value"] / (kernel_ms[n] * 1e-3), 1)
-        tot = sum(per[n]["value"] for n in cfg["patterns"])
+            per_pattern[n]["gpu_speedup"] = (None if c["value"] is None
+                                             else round(c["value"] / (kernel_ms[n] * 1e-3), 1))
+        tot = sum(per[n]["value"] or 0.0 for n in cfg["patterns"])
         cpu = {"value": round(tot, 3), "unit": "s", "cores": info["threads"], "kind": "port",
                "host": info,
                "sample": "; ".join(f"{n}: {per[n]['roots_done']} of {per[n]['roots']} random roots finished in "

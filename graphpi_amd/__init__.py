@@ -121,6 +121,7 @@ class Tuning(ctypes.Structure):
         ("slice_ms", ctypes.c_int32),
         ("no_special", ctypes.c_int32),
         ("time_limit_s", ctypes.c_double),
+        ("merge_ratio", ctypes.c_int32),
     ]
 
 
@@ -191,6 +192,10 @@ _SIGS = {
     "gp_count_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
                                       ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]),
+    "gp_count_batch_progress": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
+                                               ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(Stats)]),
     "gp_tc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]),
     "gp_kernel_plan": (ctypes.c_int, [ctypes.POINTER(SchedFlat), ctypes.POINTER(PlanInfo)]),
     "gp_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
@@ -481,6 +486,24 @@ class DeviceGraph:
         _check(lib().gp_count_batch(self._h, arr, n, ctypes.byref(tuning) if tuning else None, raws, cnts,
                                     ctypes.byref(st)))
         return [CountResult(cnts[i], raws[i], None) for i in range(n)], st
+
+    def count_batch_progress(self, scheds, begin=None, tuning=None):
+        """gp_count_batch_progress: count each schedule's tasks from begin[i]
+        (default 0) in bounded launches until done or tuning.time_limit_s
+        runs out.  Returns (done, raws, stats, finished): done[i] is the first
+        task not counted, raws[i] the u64 raw sum of tasks [begin[i],
+        done[i]); finished is True when every count is complete."""
+        n = len(scheds)
+        arr = (SchedFlat * n)(*scheds)
+        b = (ctypes.c_uint64 * n)(*(begin or [0] * n))
+        done = (ctypes.c_uint64 * n)()
+        raws = (ctypes.c_uint64 * n)()
+        st = Stats()
+        rc = lib().gp_count_batch_progress(self._h, arr, n, ctypes.byref(tuning) if tuning else None, b, done, raws,
+                                           ctypes.byref(st))
+        if rc not in (0, GP_ETIME):
+            _check(rc)
+        return list(done), list(raws), st, rc == 0
 
     def footprint(self, scheds, tuning=None):
         """Device bytes (besides the graph) a count_batch of scheds would use."""

@@ -67,6 +67,10 @@
 #define GP_SAMPLE_MIN 64   // LDS entries a sampled index needs at least
 #define GP_BITMAP_RATIO 16 // a block set goes into an LDS bitmap when its words <= this x its sorted table
 #define GP_FLAG_VMAX ((1 << 23) - 1) // flagged LDS tables pack vertex ids below this with 8 flag bits
+#define GP_MERGE_MIN 256   // merge-path intersection: the shorter list has at least this many elements
+#define GP_MERGE_TILE 256  // merged elements per merge-path tile, at most (LDS: two lists of this many)
+#define GP_MERGE_RATIO 32  // default gp_tuning.merge_ratio
+#define GP_FETCH_US 1000   // adaptive fetch size: microseconds of work a fetch aims at
 #define GP_GRAPH_CACHE 8          // instantiated launch graphs kept per call context
 
 namespace {
@@ -96,6 +100,9 @@ struct KArgs {
     int32_t hash_slots;   // LDS hash pool entries (power of two, >= 64)
     int32_t hash_lg;
     int32_t split_min;    // defer when the depth-2 loop has more candidates (INT_MAX: never)
+    int32_t merge_ratio;  // prefix builds: merge-path intersection up to this length ratio (0: never)
+    int64_t fetch_ticks;  // adaptive fetch size: wall-clock ticks a fetch aims at (0: always chunk items)
+    int64_t deadline_ticks; // first launch: stop fetching after this many ticks (0: run every item)
     unsigned long long* heavy;     // deferred: plan << 56 | task << 24 | min(candidates, 2^24 - 1)
     unsigned long long* heavy_cnt; // number deferred (may exceed heavy_cap)
     uint64_t heavy_cap;            // entries of the deferred list; the rest run in place
@@ -174,12 +181,92 @@ __device__ __forceinline__ int ilog2_ceil(int32_t x);
 __device__ void stage_sorted(int32_t* tab, int lg, const int32_t* __restrict__ g, int32_t n);
 __device__ __forceinline__ bool staged_has(const int32_t* tab, int lg, int32_t x);
 
+__device__ int32_t wave_excl_scan(int32_t x, int32_t* total);
+
+// Merge-path intersection of two sorted lists (every element below the
+// caller's bound), written sorted to out.  The wave walks the merged order
+// of S and L in tiles of T elements (T a multiple of 64, tile: 2T LDS
+// entries): it stages the next T elements of each list, finds where the
+// tile's diagonal splits them (S first on ties), and each lane merges its
+// T/64 consecutive elements of the tile from LDS.  An element x of S is a
+// hit when it equals the first element of L not yet consumed; with ties
+// taking S first that element is at most T - 1 past the tile's start in L,
+// so it is staged even when it falls in the next tile.  A lane counts its
+// hits, an exclusive scan places them, and the lane merges again to write
+// them.  Every list element is read from HBM with coalesced loads, about
+// twice (what is staged past the split is read again by the next tile),
+// where a binary search reads log2 |L| scattered elements per element of S:
+// cheaper when both lists are long and of similar length (hub lists).
+__device__ int32_t intersect_merge(const int32_t* __restrict__ Sp, int32_t ns, const int32_t* __restrict__ Lp,
+                                   int32_t nl, int32_t* __restrict__ outp, int32_t* tile, int32_t T) {
+    const gcptr S = GL(Sp), Lg = GL(Lp);
+    const gptr out = GLW(outp);
+    const int lane = lane_id();
+    const int32_t per = T / WAVE;
+    int32_t* const ts = tile;
+    int32_t* const tl = tile + T;
+    int32_t i = 0, j = 0, total = 0;
+    while (i < ns && j < nl) { // once either list is used up nothing else can match
+        for (int32_t t = lane; t < T; t += WAVE) {
+            ts[t] = i + t < ns ? S[i + t] : INT_MAX;
+            tl[t] = j + t < nl ? Lg[j + t] : INT_MAX;
+        }
+        __syncthreads();
+        const int32_t dt = min(T, (ns - i) + (nl - j));
+        // elements of S among the first d of the tile's merged order
+        auto split = [&](int32_t d) -> int32_t {
+            int32_t lo = 0, hi = d;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (ts[mid] <= tl[d - mid - 1])
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            return lo;
+        };
+        const int32_t d0 = min(dt, lane * per), d1 = min(dt, d0 + per);
+        const int32_t a0 = split(d0), ti = split(dt);
+        // k < dt keeps a + b < T, so both reads stay inside the tile; a
+        // padded INT_MAX never ties a real element (a + b < the elements left)
+        int32_t a = a0, b = d0 - a0, c = 0;
+        for (int32_t k = d0; k < d1; ++k) {
+            const int32_t sv = ts[a], lv = tl[b];
+            const bool take = sv <= lv;
+            c += (take && sv == lv) ? 1 : 0;
+            a += take ? 1 : 0;
+            b += take ? 0 : 1;
+        }
+        int32_t sum;
+        int32_t off = total + wave_excl_scan(c, &sum);
+        if (sum > 0 && c > 0) {
+            a = a0;
+            b = d0 - a0;
+            for (int32_t k = d0; k < d1; ++k) {
+                const int32_t sv = ts[a], lv = tl[b];
+                const bool take = sv <= lv;
+                if (take && sv == lv) out[off++] = sv;
+                a += take ? 1 : 0;
+                b += take ? 0 : 1;
+            }
+        }
+        total += sum;
+        i += ti;
+        j += dt - ti;
+        __syncthreads(); // every lane is done with the tile before it is refilled
+    }
+    return total;
+}
+
 // pool (LDS, pool_n entries, may be null): when the longer list, cut at ub,
 // fits, it is staged there sorted and searched in LDS instead of HBM.
+// Otherwise, with merge_ratio > 0, lists of at least GP_MERGE_MIN elements
+// whose longer one is at most merge_ratio times the shorter are merged
+// (intersect_merge, tiles in the pool) instead of searched.
 template <bool COUNT>
 __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const int32_t* __restrict__ B,
                              int32_t nb, int32_t ub, int32_t* __restrict__ outp, const int32_t* emb,
-                             int m, int32_t* pool = nullptr, int32_t pool_n = 0) {
+                             int m, int32_t* pool = nullptr, int32_t pool_n = 0, int32_t merge_ratio = 0) {
     const int lane = lane_id();
     const gptr out = GLW(outp);
     const int32_t* S = A;
@@ -199,6 +286,10 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
         if ((1 << lg) <= pool_n) {
             stage_sorted(pool, lg, Lg, nl);
             lgp = lg;
+        } else if (!COUNT && merge_ratio > 0 && ns >= GP_MERGE_MIN && nl <= (int64_t)ns * merge_ratio &&
+                   pool_n >= 2 * WAVE) {
+            if (ub != INT_MAX) ns = wave_lower_bound(S, ns, ub);
+            return intersect_merge(S, ns, Lg, nl, outp, pool, min(GP_MERGE_TILE, (pool_n / 2) & ~(WAVE - 1)));
         }
     }
     const uint64_t below_me = (1ull << lane) - 1;
@@ -499,7 +590,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             const int32_t* fp = s_ptr[f];
             const int32_t fl = s_len[f];
             if (FAITHFUL) balg += 4ull * (uint64_t)((int64_t)fl + deg);
-            len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0, s_hash, a.hash_slots);
+            len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0, s_hash, a.hash_slots,
+                                   FAITHFUL ? 0 : a.merge_ratio);
             ptr = out;
             __syncthreads(); // make the new set visible to every lane
         }
@@ -1468,30 +1560,85 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     // heavy ones.  Second launch (PARTS = true): parts of the deferred tasks,
     // one per fetch.
     const uint64_t n_items = !PARTS ? a.n_tasks : min((uint64_t)*a.heavy_cnt, a.heavy_cap) * GP_SPLIT_PARTS;
+    // [cur, cend): tasks of the current plan still to run from this fetch;
+    // [cur, fend): the whole fetch, which may run on into the next plan's items.
+    // The fetch size adapts per wave (a.fetch_ticks > 0): the first fetch
+    // takes one item, each later one as many as the last fetch's time per
+    // item says fit in fetch_ticks, at most a.chunk.  Cheap tasks then come
+    // a.chunk at a time (few atomics on the shared counter), expensive ones
+    // one at a time (no wave holds a run of expensive tasks at the end).
+    // (the fetch's end, size and start time live in LDS: they are touched
+    // once per fetch, and registers held across the whole loop spill)
+    // With a.deadline_ticks > 0 a wave stops fetching once that many ticks
+    // have passed since it started (after its first fetch, so every launch
+    // makes progress).  Items are handed out in order from one counter and a
+    // fetched item is always finished, so the launch then has counted
+    // exactly the items below the counter's final value: a prefix, which the
+    // host resumes from.
+    __shared__ uint64_t s_fend, s_tfetch, s_tstart;
+    __shared__ int32_t s_chunk;
+    if (lane == 0) {
+        s_fend = 0;
+        s_tfetch = 0;
+        s_tstart = wall_clock64();
+        s_chunk = a.fetch_ticks > 0 ? 1 : a.chunk;
+    }
+    __syncthreads();
     uint64_t cur = 0, cend = 0;
     for (;;) {
         uint64_t t;
         if (!PARTS) {
-            if (cur >= cend) {
-                if (EXT == GP_EXT_PAIR) pv_clear(); // the dense array is zero between fetches
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)a.chunk);
-                base = first_lane_u64(base);
-                if (base >= n_items) break;
-                // the plan owning this chunk; a chunk never spans two plans
+            bool done = false;
+            while (cur >= cend) {
+                if (cur >= s_fend) {
+                    if (EXT == GP_EXT_PAIR) pv_clear(); // the dense array is zero between fetches
+                    if (a.deadline_ticks > 0 && s_fend > 0 &&
+                        first_lane_u64(wall_clock64()) - s_tstart > (uint64_t)a.deadline_ticks) {
+                        done = true;
+                        break;
+                    }
+                    int32_t chunk_now = s_chunk;
+                    if (a.fetch_ticks > 0 && s_fend > 0) {
+                        const uint64_t el = wall_clock64() - s_tfetch;
+                        const uint64_t per = max(el / (uint64_t)chunk_now, (uint64_t)1);
+                        chunk_now = (int32_t)min((uint64_t)a.chunk, max((uint64_t)a.fetch_ticks / per, (uint64_t)1));
+                        chunk_now = __builtin_amdgcn_readfirstlane(chunk_now);
+                    }
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(a.counter, (unsigned long long)chunk_now);
+                    base = first_lane_u64(base);
+                    if (base >= n_items) {
+                        done = true;
+                        break;
+                    }
+                    __syncthreads(); // every lane has read the old fetch state
+                    if (lane == 0) {
+                        s_chunk = chunk_now;
+                        s_fend = min((uint64_t)base + (uint64_t)chunk_now, n_items);
+                        if (a.fetch_ticks > 0) s_tfetch = wall_clock64();
+                    }
+                    __syncthreads();
+                    cur = base;
+                }
+                // the plan owning item cur
                 int p = 0;
                 if (MULTI) {
-                    while (p + 1 < a.nplans && a.task_off[p + 1] <= base) ++p;
+                    while (p + 1 < a.nplans && a.task_off[p + 1] <= cur) ++p;
                     if (p != cur_plan) {
                         flush();
                         set_plan(p);
                     }
                 }
-                cur = base;
-                cend = min((uint64_t)base + (uint64_t)a.chunk, (uint64_t)(a.task_off[p] + a.task_cnt[p]));
-                if (cur >= cend) continue; // padding
-
+                const uint64_t pend = a.task_off[p] + a.task_cnt[p];
+                const uint64_t fend = s_fend;
+                if (cur < pend) {
+                    cend = min(fend, pend);
+                } else { // padding after the plan's tasks: on to the next plan's items
+                    cur = min(fend, (uint64_t)a.task_off[p + 1]);
+                    cend = cur;
+                }
             }
+            if (done) break;
             t = a.task_first[cur_plan] + (cur - a.task_off[cur_plan]);
             ++cur;
             ++taken;
@@ -1684,6 +1831,7 @@ struct CallCtx {
 struct gp_graph {
     int device = 0;
     int num_cu = 0;
+    int wall_khz = 100000; // wall-clock ticks per millisecond (wall_clock64 in kernels)
     int32_t v_cnt = 0;
     uint32_t e_cnt = 0;
     int32_t max_deg = 0, max_deg2 = 0;
@@ -1787,7 +1935,8 @@ struct Launch {
     std::vector<uint64_t> task_off, task_first, task_cnt;
     uint64_t n_tasks = 0;
     int64_t waves = 0;
-    int chunk = 1;
+    int chunk = 1;            // tasks per fetch, or the most a fetch takes when fetch_ticks > 0
+    int64_t fetch_ticks = 0;  // adaptive fetch size: ticks a fetch aims at (0: fixed chunk)
     int64_t slot_stride = 0, tmp_stride = 0, bslot_stride = 0, wave_stride = 0;
     size_t scratch_bytes = 0;
     int hash_slots = 0, hash_lg = 0;
@@ -1804,7 +1953,10 @@ struct Launch {
     }
 };
 
-int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, Launch* out) {
+// rb, re (optional): plan p counts tasks [rb[p], re[p]) instead of the
+// tuning's range (which applies to a single plan only)
+int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, Launch* out,
+            const uint64_t* rb = nullptr, const uint64_t* re = nullptr) {
     if (nplans < 1 || nplans > GP_MAX_BATCH) return gp_fail(GP_ERANGE, "schedule count out of range");
     const bool faithful = tuning && tuning->collect_balg;
     std::vector<KPlan> kps((size_t)nplans);
@@ -1851,7 +2003,10 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     for (int p = 0; p < nplans; ++p) {
         const uint64_t all_tasks = kps[p].root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
         uint64_t t_begin = 0, t_end = all_tasks;
-        if (tuning && (tuning->task_begin != 0 || tuning->task_end > 0)) {
+        if (rb && re) {
+            t_begin = std::min(rb[p], all_tasks);
+            t_end = std::min(re[p], all_tasks);
+        } else if (tuning && (tuning->task_begin != 0 || tuning->task_end > 0)) {
             if (nplans != 1) return gp_fail(GP_EINVAL, "task ranges apply to a single schedule");
             if (tuning->task_begin < 0 || (tuning->task_end > 0 && tuning->task_end < tuning->task_begin))
                 return gp_fail(GP_EINVAL, "bad task range");
@@ -1863,18 +2018,31 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     }
     uint64_t real_tasks = 0;
     for (int p = 0; p < nplans; ++p) real_tasks += task_cnt[p];
-    // tasks per work fetch: 16, fewer when there are under 32 chunks per wave
-    // (small graphs: the last chunks then decide the kernel's length)
+    // tasks per work fetch: at most 16, fewer when a plan has under 32
+    // fetches of 16 per wave (small graphs: the last fetches then decide the
+    // kernel's length); within that cap each wave sizes its fetches for about
+    // GP_FETCH_US of work from the time its last fetch took.  Measured on
+    // R-MAT s22: the 4-clique (about 30 us of a wave per task) took 317 ms in
+    // fetches of 16 and 847 ms in fetches of 1 (6,144 waves queue on one
+    // atomic counter); the 5-clique on the Chung-Lu 3M graph (about 17 ms
+    // per task) took 10.3 s on a slice in fetches of 1 and 12.9 s in fetches
+    // of 16; all 133 5- and 6-vertex motifs as one batch on R-MAT s12 took
+    // 11.5 s in fetches of 1 and 23.4 s in fetches of 16.
     // (512 for a single plan whose pair level is grouped by v1: its counts
     // carry over between the tasks of one fetch; measured on R-MAT s16-s20,
     // 256 beat 64 and the default by 1.2-6x despite the coarser balance, and
     // 512 beat 256 by 6 % at s20)
     const bool grouped = nplans == 1 && kps[0].pair_v1;
+    const uint64_t per_plan = real_tasks / (uint64_t)nplans;
     const int chunk = tuning && tuning->chunk > 0
                           ? tuning->chunk
                           : grouped ? 512
-                                    : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, real_tasks / (uint64_t)(waves * 32)));
-    // each plan's items start on a chunk boundary, so no fetch spans two plans
+                                    : (int)std::max<uint64_t>(1, std::min<uint64_t>(16, per_plan / (uint64_t)(waves * 32)));
+    const int64_t fetch_ticks =
+        (tuning && tuning->chunk > 0) || grouped ? 0 : (int64_t)g->wall_khz * GP_FETCH_US / 1000;
+    // each plan's items start on a chunk boundary (a fetch of a fixed chunk
+    // then never spans two plans; an adaptive fetch may, and the kernel walks
+    // on into the next plan's items)
     for (int p = 0; p < nplans; ++p)
         task_off[p + 1] = task_off[p] + (task_cnt[p] + (uint64_t)chunk - 1) / (uint64_t)chunk * (uint64_t)chunk;
     const uint64_t n_tasks = task_off[nplans];
@@ -1928,6 +2096,7 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     out->n_tasks = n_tasks;
     out->waves = waves;
     out->chunk = chunk;
+    out->fetch_ticks = fetch_ticks;
     out->slot_stride = slot_stride;
     out->tmp_stride = tmp_stride;
     out->bslot_stride = bslot_stride;
@@ -2261,10 +2430,15 @@ int run_house(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
 // Count nplans schedules over one graph in one persistent launch (plus the
 // launch of deferred parts).  Results are per schedule; a task range in the
 // tuning is only accepted for a single schedule.
+// rb, re: per-plan task ranges (see prepare).  deadline_ms > 0: the task
+// launch stops fetching after that long (the kernel's deadline), and
+// plan_done[p] receives how many of plan p's tasks, from its first, were
+// counted (all of them without a deadline).
 int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, uint64_t* raw,
-              int64_t* count, gp_stats* stats) {
+              int64_t* count, gp_stats* stats, const uint64_t* rb = nullptr, const uint64_t* re = nullptr,
+              double deadline_ms = 0, uint64_t* plan_done = nullptr) {
     // schedule-specialised kernels for single counts of shapes they cover
-    if (nplans == 1 && !(tuning && (tuning->collect_balg || tuning->no_special))) {
+    if (nplans == 1 && !rb && deadline_ms <= 0 && !(tuning && (tuning->collect_balg || tuning->no_special))) {
         const int shape = gp_sched_shape(&scheds[0]);
         if (shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL)
             return run_house(g, scheds, tuning, shape == GP_SHAPE_HOUSE_ALL, raw, count, stats);
@@ -2273,7 +2447,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     }
     Launch ln;
     {
-        int rc = prepare(g, scheds, nplans, tuning, &ln);
+        int rc = prepare(g, scheds, nplans, tuning, &ln, rb, re);
         if (rc != GP_OK) return rc;
     }
     HIP_TRY(hipSetDevice(g->device));
@@ -2324,7 +2498,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
                 sizeof(uint64_t) * (size_t)nplans);
     std::memcpy(meta.data() + plans_bytes + sizeof(uint64_t) * (2 * (size_t)nplans + 1), task_cnt.data(),
                 sizeof(uint64_t) * (size_t)nplans);
-    KArgs a;
+    KArgs a{};
     a.plans = reinterpret_cast<const KPlan*>(c->d_meta);
     a.nplans = nplans;
     a.task_off = reinterpret_cast<const uint64_t*>(c->d_meta + plans_bytes);
@@ -2335,6 +2509,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.src = g->d_src;
     a.v_cnt = g->v_cnt;
     a.chunk = chunk;
+    a.fetch_ticks = ln.fetch_ticks;
     a.n_tasks = n_tasks;
     a.counter = c->d_cnt;
     a.counter2 = c->d_cnt + 1;
@@ -2351,6 +2526,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.hash_lg = hash_lg;
     a.probe_pct = tuning && tuning->probe_pct > 0 ? tuning->probe_pct : 100;
     a.bitmap_ratio = !tuning || tuning->bitmap_ratio == 0 ? GP_BITMAP_RATIO : std::max(0, tuning->bitmap_ratio);
+    a.merge_ratio = !tuning || tuning->merge_ratio == 0 ? GP_MERGE_RATIO : std::max(0, tuning->merge_ratio);
+    a.deadline_ticks = deadline_ms > 0 ? std::max<int64_t>(1, (int64_t)(deadline_ms * g->wall_khz)) : 0;
     a.dense = dense_bytes ? c->d_dense : nullptr;
     a.dense_stride = dense_stride;
     a.clocks = c->d_clocks;
@@ -2419,6 +2596,12 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         return gp_fail((int)e, std::string("count kernel: ") + hipGetErrorString(e));
     }
     release_ctx(g, c);
+    if (plan_done) {
+        // the items below the counter's final value were counted
+        const uint64_t upto = a.deadline_ticks > 0 ? std::min<uint64_t>(host_out[0], n_tasks) : n_tasks;
+        for (int p = 0; p < nplans; ++p)
+            plan_done[p] = upto > task_off[p] ? std::min<uint64_t>(upto - task_off[p], task_cnt[p]) : 0;
+    }
     uint64_t balg_total = 0, tasks_total = 0;
     for (int p = 0; p < nplans; ++p) {
         const uint64_t sum = host_out[3 + 3 * p];
@@ -2613,6 +2796,11 @@ extern "C" int gp_graph_upload(const int32_t* edge, const uint32_t* vertex, int3
     gp_graph* g = new gp_graph();
     HIP_TRY(hipGetDevice(&g->device));
     HIP_TRY(hipDeviceGetAttribute(&g->num_cu, hipDeviceAttributeMultiprocessorCount, g->device));
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) == hipSuccess && khz > 0)
+            g->wall_khz = khz;
+    }
     g->v_cnt = v_cnt;
     g->e_cnt = e_cnt;
     for (int32_t v = 0; v < v_cnt; ++v) {
@@ -2670,10 +2858,15 @@ double wall_seconds() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// One schedule's count as consecutive launches over task ranges
-// (GpRangePlan): no single launch runs for long, and a host-side time limit
-// is checked between launches.  Raw sums of disjoint ranges add up exactly
-// (u64 wrap-around), so the result is the one-launch result.
+// One schedule's count as consecutive bounded launches, so that no single
+// launch runs for long and a host-side time limit is checked between them.
+// The generic kernel gets a deadline of slice_ms per launch: each launch
+// counts a prefix of the remaining tasks and the next starts where it
+// stopped, so a count that fits one slice is one launch and a long one
+// becomes launches of about slice_ms whatever the cost of its tasks.  The
+// House kernel (a root per fetch) is sliced by GpRangePlan instead.  Raw
+// sums of consecutive ranges add up exactly (u64 wrap-around), so the
+// result is the one-launch result.
 int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, uint64_t* raw_sum,
                  int64_t* count, gp_stats* stats) {
     KPlan kp;
@@ -2690,19 +2883,13 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
     if (shape == GP_SHAPE_TRI_EDGE_TRI && t.task_begin == 0 && t.task_end <= 0)
         return run_tri_edge_tri(g, sched, raw_sum, count, stats);
     const bool per_root = shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL;
-    GpRangePlan plan(begin, end, t.slice_ms, t.time_limit_s, wall_seconds(), per_root ? 4 : 256);
-    if (plan.single()) return run_plans(g, sched, 1, &t, raw_sum, count, stats);
+    const double t0 = wall_seconds();
+    if (t.slice_ms < 0 || end <= begin) return run_plans(g, sched, 1, &t, raw_sum, count, stats);
+    const double slice = t.slice_ms > 0 ? (double)t.slice_ms : 2000.0;
     gp_stats acc{};
-    uint64_t raw = 0, b = 0, e = 0;
-    while (plan.next(&b, &e)) {
-        gp_tuning sub = t;
-        sub.task_begin = (int64_t)b;
-        sub.task_end = (int64_t)e;
-        uint64_t r = 0;
-        gp_stats st{};
-        rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st);
-        if (rc != GP_OK) return rc;
-        raw += r;
+    uint64_t raw = 0, cur = begin;
+    bool timed_out = false;
+    auto add = [&](const gp_stats& st) {
         const double before_ms = acc.kernel_ms;
         acc.kernel_ms += st.kernel_ms;
         acc.balg_bytes += st.balg_bytes;
@@ -2715,17 +2902,50 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
         acc.finish_p50_ms = before_ms + st.finish_p50_ms;
         acc.finish_p90_ms = before_ms + st.finish_p90_ms;
         acc.finish_max_ms = before_ms + st.finish_max_ms;
-        plan.finished(e, st.kernel_ms, wall_seconds());
+        ++acc.launches;
+    };
+    if (per_root) {
+        GpRangePlan plan(begin, end, t.slice_ms, t.time_limit_s, t0, 4);
+        uint64_t b = 0, e = 0;
+        while (plan.next(&b, &e)) {
+            gp_tuning sub = t;
+            sub.task_begin = (int64_t)b;
+            sub.task_end = (int64_t)e;
+            uint64_t r = 0;
+            gp_stats st{};
+            rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st);
+            if (rc != GP_OK) return rc;
+            raw += r;
+            add(st);
+            plan.finished(e, st.kernel_ms, wall_seconds());
+        }
+        cur = plan.cur;
+        timed_out = plan.timed_out;
+    } else {
+        while (cur < end) {
+            gp_tuning sub = t;
+            sub.task_begin = (int64_t)cur;
+            sub.task_end = (int64_t)end;
+            uint64_t r = 0, did = 0;
+            gp_stats st{};
+            rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st, nullptr, nullptr, slice, &did);
+            if (rc != GP_OK) return rc;
+            raw += r;
+            add(st);
+            cur += did;
+            if (cur < end && t.time_limit_s > 0 && wall_seconds() - t0 > t.time_limit_s) {
+                timed_out = true;
+                break;
+            }
+        }
     }
-    acc.launches = plan.launches;
-    acc.finished = plan.timed_out ? 0 : 1;
+    acc.finished = timed_out ? 0 : 1;
     acc.tasks_total = end > begin ? end - begin : 0;
     if (raw_sum) *raw_sum = raw;
     if (stats) *stats = acc;
-    if (plan.timed_out)
+    if (timed_out)
         return gp_fail(GP_ETIME, "time limit of " + std::to_string(t.time_limit_s) + " s reached after " +
-                                     std::to_string(plan.cur - begin) + " of " + std::to_string(end - begin) +
-                                     " tasks");
+                                     std::to_string(cur - begin) + " of " + std::to_string(end - begin) + " tasks");
     if (count) count[0] = (int64_t)raw / (sched->redundancy ? sched->redundancy : 1);
     return GP_OK;
 }
@@ -2820,7 +3040,7 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     std::memcpy(meta, ln.kps.data(), plans_bytes);
     const uint64_t layout[4] = {ln.task_off[0], ln.task_off[1], ln.task_first[0], ln.task_cnt[0]};
     std::memcpy(meta + plans_bytes, layout, sizeof(layout));
-    KArgs a;
+    KArgs a{};
     std::memset(&a, 0, sizeof(a));
     a.plans = reinterpret_cast<const KPlan*>(base);
     a.nplans = 1;
@@ -2832,6 +3052,7 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     a.src = g->d_src;
     a.v_cnt = g->v_cnt;
     a.chunk = ln.chunk;
+    a.fetch_ticks = ln.fetch_ticks;
     a.n_tasks = ln.n_tasks;
     unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + off_cnt);
     a.counter = cnt;
@@ -2849,6 +3070,7 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     a.hash_lg = ln.hash_lg;
     a.probe_pct = t.probe_pct > 0 ? t.probe_pct : 100;
     a.bitmap_ratio = t.bitmap_ratio == 0 ? GP_BITMAP_RATIO : std::max(0, t.bitmap_ratio);
+    a.merge_ratio = t.merge_ratio == 0 ? GP_MERGE_RATIO : std::max(0, t.merge_ratio);
     a.dense = nullptr;
     a.dense_stride = 0;
     unsigned long long* clk = reinterpret_cast<unsigned long long*>(base + off_clk);
@@ -2964,6 +3186,98 @@ extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, in
             stats->finish_max_ms = stats->kernel_ms - st.kernel_ms + st.finish_max_ms;
         }
     }
+    return GP_OK;
+}
+
+// gp_count_batch over part of each schedule's tasks: launches over the
+// schedules' remaining tasks laid end to end (one class of LDS size at a
+// time, as in gp_count_batch), each with a kernel deadline of slice_ms (or
+// what is left of the time limit), so that each counts a prefix of those
+// tasks; the time limit is checked between launches.  After a launch every
+// schedule's count is known up to a task index: the next launch, or the
+// caller, resumes from there, and raw sums of consecutive ranges add up.
+extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* scheds, int32_t n_scheds,
+                                       const gp_tuning* tuning, const uint64_t* begin, uint64_t* done,
+                                       uint64_t* raw_sums, gp_stats* stats) {
+    if (!gc || !scheds || !done || !raw_sums) return gp_fail(GP_EINVAL, "null argument");
+    if (n_scheds < 1 || n_scheds > GP_MAX_BATCH) return gp_fail(GP_ERANGE, "schedule count out of range");
+    gp_graph* g = const_cast<gp_graph*>(gc);
+    gp_tuning t{};
+    if (tuning) t = *tuning;
+    if (t.task_begin != 0 || t.task_end > 0) return gp_fail(GP_EINVAL, "task ranges come from begin[]");
+    if (t.collect_balg) return gp_fail(GP_EINVAL, "the faithful loop nest is not run in progress mode");
+    const double t0 = wall_seconds();
+    const int n = n_scheds;
+    std::vector<uint64_t> total((size_t)n), b((size_t)n);
+    std::vector<std::vector<int> > groups(4);
+    for (int i = 0; i < n; ++i) {
+        KPlan kp;
+        int rc = gp_make_kplan(&scheds[i], false, !t.no_batch, !t.no_pair, &kp);
+        if (rc != GP_OK) return rc;
+        total[i] = kp.root_mode ? (uint64_t)g->v_cnt : (uint64_t)g->e_cnt;
+        b[i] = begin ? std::min(begin[i], total[i]) : 0;
+        done[i] = b[i];
+        raw_sums[i] = 0;
+        groups[kp.pair ? 0 : kp.batched ? (kp.nblocks <= 3 ? 1 : 2) : 3].push_back(i);
+    }
+    gp_stats acc{};
+    bool timed_out = false;
+    const double slice = t.slice_ms > 0 ? (double)t.slice_ms : 2000.0;
+    for (const std::vector<int>& grp : groups) {
+        for (;;) {
+            // the class's schedules that still have tasks, in order
+            std::vector<gp_sched_flat> sub;
+            std::vector<uint64_t> rb, re;
+            std::vector<int> who;
+            for (int i : grp) {
+                if (done[i] >= total[i]) continue;
+                who.push_back(i);
+                sub.push_back(scheds[i]);
+                rb.push_back(done[i]);
+                re.push_back(total[i]);
+            }
+            if (who.empty()) break;
+            double left_ms = slice;
+            if (t.time_limit_s > 0) {
+                left_ms = (t.time_limit_s - (wall_seconds() - t0)) * 1000.0;
+                if (left_ms <= 0) {
+                    timed_out = true;
+                    break;
+                }
+                left_ms = std::min(left_ms, slice);
+            }
+            std::vector<uint64_t> raw(sub.size()), did(sub.size());
+            gp_stats st{};
+            int rc = run_plans(g, sub.data(), (int)sub.size(), &t, raw.data(), nullptr, &st, rb.data(), re.data(),
+                               left_ms, did.data());
+            if (rc != GP_OK) return rc;
+            for (size_t k = 0; k < who.size(); ++k) {
+                raw_sums[who[k]] += raw[k];
+                done[who[k]] += did[k];
+            }
+            const double before_ms = acc.kernel_ms;
+            acc.kernel_ms += st.kernel_ms;
+            acc.tasks += st.tasks;
+            acc.split_tasks += st.split_tasks;
+            acc.inplace_tasks += st.inplace_tasks;
+            acc.waves = std::max(acc.waves, st.waves);
+            acc.finish_p50_ms = before_ms + st.finish_p50_ms;
+            acc.finish_p90_ms = before_ms + st.finish_p90_ms;
+            acc.finish_max_ms = before_ms + st.finish_max_ms;
+            ++acc.launches;
+        }
+        if (timed_out) break;
+    }
+    bool all_done = true;
+    for (int i = 0; i < n; ++i) {
+        acc.tasks_total += total[i] - b[i];
+        all_done = all_done && done[i] == total[i];
+    }
+    acc.finished = all_done ? 1 : 0;
+    if (stats) *stats = acc;
+    if (!all_done)
+        return gp_fail(GP_ETIME, std::string("time limit of ") + std::to_string(t.time_limit_s) +
+                                     " s reached before every schedule's count was complete");
     return GP_OK;
 }
 

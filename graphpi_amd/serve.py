@@ -5,13 +5,21 @@ requests (a pattern, its planner settings and a graph name).  Worker slots
 take requests continuously: a slot that frees up immediately gathers the
 requests waiting for one graph (oldest first), up to a batch size and within
 the device-memory budget (each batch's footprint comes from
-gp_count_footprint), and counts them in one gp_count_batch launch.  Slots run
-on their own threads and HIP streams, so a batch that arrives while others are
-on the GPU starts as soon as a slot and memory are free and its kernels overlap
-with theirs.  Every result is the exact count: batching and concurrency change
-when a request runs, never what it returns.  A tuning with use_graph = 1
-runs each slot's launches as hipGraphs (upload, counter reset, kernels and
-read-back as one graph launch, replayed while the shape repeats).
+gp_count_footprint), and counts them together with
+gp_count_batch_progress.  Slots run on their own threads and HIP streams, so
+a batch that arrives while others are on the GPU starts as soon as a slot and
+memory are free and its kernels overlap with theirs.
+
+Counts run in time slices: a batch call stops after about quantum_s (its
+launches stop fetching at a deadline, so each has counted a prefix of every
+request's tasks), finished requests are answered, and the others go back to
+the queue with their progress and continue in a later batch from the task
+where they stopped.  A long count therefore never holds a slot for long and
+short requests that arrive behind it are not stuck.  A request still
+unfinished when its deadline (seconds after arrival) has passed fails with
+RequestTimeout.  Every answered result is the exact count: the raw sums of
+consecutive task ranges add up, so slicing, batching and concurrency change
+when a request runs, never what it returns.
 
 Schedules are built on the host (the planner) and cached per (graph, pattern,
 settings), as a server would cache them per canonical pattern.
@@ -25,21 +33,40 @@ import numpy as np
 from . import DeviceGraph, build_schedule, device_memory
 
 
-class Request:
-    __slots__ = ("rid", "graph", "adj", "settings", "t_arrival", "t_start", "t_done", "count", "raw", "error",
-                 "sched", "_event")
+class RequestTimeout(TimeoutError):
+    """The request's deadline passed before its count was complete; .done of
+    .total tasks had been counted."""
 
-    def __init__(self, rid, graph, adj, settings=(1, 1, 1), t_arrival=None):
+    def __init__(self, rid, done, total):
+        super().__init__(f"request {rid}: deadline passed after {done} of {total} tasks")
+        self.done = done
+        self.total = total
+
+
+class Request:
+    __slots__ = ("rid", "graph", "adj", "settings", "deadline", "t_arrival", "t_start", "t_done", "count", "raw",
+                 "error", "sched", "total", "done", "raw_acc", "slices", "_event")
+
+    def __init__(self, rid, graph, adj, settings=(1, 1, 1), t_arrival=None, deadline=None):
         self.rid = rid
         self.graph = graph
         self.adj = np.asarray(adj, dtype=np.int32)
         self.settings = tuple(settings)
+        self.deadline = deadline  # seconds after arrival; None: the server's default
         self.t_arrival = time.perf_counter() if t_arrival is None else t_arrival
         self.t_start = self.t_done = None
         self.count = self.raw = None
         self.error = None
         self.sched = None
+        self.total = 0      # tasks of the count
+        self.done = 0       # tasks counted so far (a prefix)
+        self.raw_acc = 0    # u64 raw sum of those tasks
+        self.slices = 0     # batch calls the request took part in
         self._event = threading.Event()
+
+    @property
+    def timed_out(self):
+        return isinstance(self.error, RequestTimeout)
 
     def wait(self, timeout=None):
         if not self._event.wait(timeout):
@@ -57,13 +84,18 @@ class CountServer:
     """graphs: {name: HostGraph}.  slots: concurrent batches on the device.
     max_batch: schedules per launch.  mem_budget: device bytes the in-flight
     batches may use besides the graphs (a batch larger than the budget still
-    runs, alone)."""
+    runs, alone).  quantum_s: length of a batch call's time slice (None: a
+    batch runs until every count in it is complete).  deadline_s: default
+    deadline of a request, in seconds after its arrival (None: none)."""
 
     MAX_BATCH = 256  # schedules gp_count_batch takes per call
 
-    def __init__(self, graphs, slots=4, max_batch=32, mem_budget=None, tuning=None, device_factory=None):
+    def __init__(self, graphs, slots=4, max_batch=32, mem_budget=None, tuning=None, device_factory=None,
+                 quantum_s=0.1, deadline_s=None):
         if not 1 <= max_batch <= self.MAX_BATCH:
             raise ValueError(f"max_batch must be 1..{self.MAX_BATCH}")
+        self._quantum = quantum_s
+        self._deadline = deadline_s
         make = device_factory or DeviceGraph
         self._dev = {}
         self._host = {}
@@ -103,6 +135,9 @@ class CountServer:
             raise KeyError(f"unknown graph {req.graph}")
         try:
             req.sched = self.schedule_for(req.graph, req.adj, req.settings)
+            req.total = self._dev[req.graph].count_tasks(req.sched)
+            if req.deadline is None:
+                req.deadline = self._deadline
         except Exception as e:
             req.error = e
             req.t_done = time.perf_counter()
@@ -157,7 +192,10 @@ class CountServer:
             else:
                 rest.append(r)
         self._pending = rest
-        ok = batch
+        # requests not yet started go first: a batch call lays the counts'
+        # remaining tasks end to end in this order, so within its time slice
+        # the new (mostly short) requests finish before resumed long ones
+        ok = sorted(batch, key=lambda r: (r.done > 0, r.t_arrival))
         scheds = [r.sched for r in ok]
         dev = self._dev[graph]
         try:
@@ -181,7 +219,19 @@ class CountServer:
         self._in_flight += fp
         return (graph, ok, scheds, fp)
 
+    def _tuning_for_slice(self):
+        from . import Tuning
+        t = Tuning()
+        if self._tuning is not None:
+            import ctypes
+            ctypes.memmove(ctypes.byref(t), ctypes.byref(self._tuning), ctypes.sizeof(Tuning))
+        if self._quantum:
+            t.time_limit_s = self._quantum
+            t.slice_ms = max(1, int(self._quantum * 1000.0))
+        return t
+
     def _run(self):
+        tun = self._tuning_for_slice()
         while True:
             with self._cv:
                 item = None
@@ -194,23 +244,44 @@ class CountServer:
             graph, reqs, scheds, fp = item
             t0 = time.perf_counter()
             for r in reqs:
-                r.t_start = t0
+                if r.t_start is None:
+                    r.t_start = t0
             try:
-                res, _ = self._dev[graph].count_batch(scheds, self._tuning)
+                done, raws, _, _ = self._dev[graph].count_batch_progress(scheds, [r.done for r in reqs], tun)
                 err = None
             except Exception as e:
-                res, err = None, e
+                done = raws = None
+                err = e
             t1 = time.perf_counter()
+            finished, again = [], []
+            for i, r in enumerate(reqs):
+                r.slices += 1
+                if err is not None:
+                    r.error = err
+                    finished.append(r)
+                    continue
+                r.raw_acc = (r.raw_acc + raws[i]) % (1 << 64)
+                r.done = done[i]
+                if r.done >= r.total:
+                    signed = r.raw_acc - (1 << 64) if r.raw_acc >= (1 << 63) else r.raw_acc
+                    red = r.sched.redundancy or 1
+                    q = abs(signed) // abs(red)
+                    r.count = q if (signed >= 0) == (red > 0) else -q
+                    r.raw = r.raw_acc
+                    finished.append(r)
+                elif r.deadline is not None and t1 - r.t_arrival > r.deadline:
+                    r.error = RequestTimeout(r.rid, r.done, r.total)
+                    finished.append(r)
+                else:
+                    again.append(r)
             with self._cv:
                 self._in_flight -= fp
                 self.batches.append((graph, len(reqs), fp, t1 - t0))
+                # unfinished counts continue in a later batch, behind what waits now
+                self._pending.extend(again)
                 self._cv.notify_all()
-            for i, r in enumerate(reqs):
+            for r in finished:
                 r.t_done = t1
-                if err is None:
-                    r.count, r.raw = res[i].count, res[i].raw
-                else:
-                    r.error = err
                 r._event.set()
 
 

@@ -136,6 +136,11 @@ typedef struct gp_tuning {
                                time has passed and return GP_ETIME (the reference's
                                Graph::max_running_time, include/graph.h:12); every launch
                                still runs to its end.  0 = no limit */
+    int32_t merge_ratio;    /* prefix-set builds: when the longer list does not fit the LDS pool,
+                               the shorter has at least 256 elements and the longer is at most
+                               this many times as long, the two are merged (merge path, tiles
+                               staged in LDS) instead of searching the longer one per element;
+                               0 = default (32), < 0 = never.  Counts are identical either way */
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */
@@ -217,6 +222,25 @@ GP_API int gp_count_noalloc(const gp_graph* g, const gp_sched_flat* sched, const
  * many-pattern drivers (MotifGenerator output, all_schedule_test.cpp). */
 GP_API int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
                           const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats);
+
+/* gp_count_batch over part of each schedule's tasks, resumable.  Schedule i
+ * counts its tasks from begin[i] (NULL: from 0) as bounded launches over the
+ * schedules' remaining tasks laid end to end (gp_tuning.slice_ms sizes them;
+ * with gp_tuning.time_limit_s > 0 the first launch of each LDS class takes
+ * one task per persistent wave).  Between launches the time limit is
+ * checked; every launch runs to its end.  On return done[i] is the first
+ * task of schedule i not counted (gp_count_tasks when its count is
+ * complete) and raw_sums[i] the u64 raw sum of its tasks [begin[i], done[i]).
+ * Raw sums of consecutive task ranges add up, so a caller resumes a count by
+ * passing done[] back as begin[] and adding the raw sums; the count is then
+ * (int64)raw / redundancy, as gp_count returns it.  Returns GP_OK when every
+ * schedule's count is complete, GP_ETIME when the limit stopped it first
+ * (done[] and raw_sums[] are valid either way).  gp_tuning task ranges and
+ * collect_balg are rejected (GP_EINVAL).  No reference counterpart: it lets
+ * a serving loop give long counts time slices (graphpi_amd/serve.py). */
+GP_API int gp_count_batch_progress(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
+                                   const gp_tuning* tuning, const uint64_t* begin, uint64_t* done,
+                                   uint64_t* raw_sums, gp_stats* stats);
 
 /* Device memory (bytes, besides the uploaded graph) that gp_count_batch with
  * the same arguments would use: per-wave scratch, plans, counters, stamps

@@ -609,3 +609,51 @@ def test_tri_edge_tri_closed_form_matches(graph, wiki_vote_path):
         assert (got.raw, got.count) == (want["raw"], want["count"])
     # task ranges take the generic kernel and still add up to the same raw sum
     assert dev.count_sliced(s, 5).raw == got.raw
+
+
+@pytest.mark.parametrize("graph", ["dense", "rmat15"])
+def test_merge_path_intersections_match(graph):
+    """Prefix builds by merge path (tiles staged in LDS) against the
+    per-element binary search (merge_ratio < 0) and the oracle.  The dense
+    graph with a small LDS pool (64-element tiles) merges most of its
+    depth-1 and depth-2 builds; R-MAT scale 15 with the default pool merges
+    the hub-against-hub builds only."""
+    if graph == "dense":
+        host = _random_graph(600, 0.5, 41)
+        names = ["4clique", "5clique", "house", "qg3", "rectangle"]
+        extra = dict(lds_stage=64)
+    else:
+        host = gp.HostGraph.rmat(15, 16, 0.57, 0.19, 0.19, 4)
+        names = ["4clique", "5clique", "house"]
+        extra = {}
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    search = _generic(merge_ratio=-1, **extra)
+    merge = _generic(merge_ratio=1 << 20, **extra)
+    for name in names:
+        s = _sched(host, gp.PATTERNS[name], 1, 1, 1, tri)
+        want = ob.pattern_matching(host, s, threads=16)
+        a, b = dev.count(s, search), dev.count(s, merge)
+        print(f"{graph} {name}: search {a.stats.kernel_ms:.2f} ms, merge {b.stats.kernel_ms:.2f} ms", flush=True)
+        assert (a.raw, a.count) == (b.raw, b.count) == (want["raw"], want["count"]), name
+        assert dev.count(s).raw == want["raw"]  # the default ratio
+
+
+def test_motif_batch_rmat_ef16_matches_oracle():
+    """The batched configuration's workload at a scale the oracle finishes in
+    seconds: all 133 connected 5- and 6-vertex patterns (MotifGenerator
+    order) over R-MAT scale 8, edge factor 16, seed 4, in one gp_count_batch
+    call; every count against the oracle's restatement of the reference loop
+    nest and against gp_count alone (which takes the specialised House
+    kernel where the batch takes the generic one)."""
+    host = gp.HostGraph.rmat(8, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = _tc(host)
+    scheds = _motif_schedules(host, (5, 6), tri)
+    assert len(scheds) == 133
+    res, st = dev.count_batch(scheds)
+    for i, (s, r) in enumerate(zip(scheds, res)):
+        want = ob.pattern_matching(host, s, threads=16)
+        alone = dev.count(s)
+        assert (r.raw, r.count) == (want["raw"], want["count"]) == (alone.raw, alone.count), i
+    print(f"133 motifs on R-MAT s8 ef16: batch {st.kernel_ms:.1f} ms", flush=True)

@@ -11,8 +11,12 @@ from graphpi_amd.serve import CountServer, Request, percentile, poisson_arrivals
 
 
 class FakeDevice:
-    """count_batch returns a value derived from each schedule alone, takes
-    time, and records how much footprint was in flight at once."""
+    """A count over TASKS tasks whose raw sum is unit(s) x redundancy per
+    task (so the count alone is value(s) = TASKS x unit(s)).  count_batch_progress
+    advances each schedule by step(s) tasks per call, so some requests take
+    several time slices; it takes time and records how much footprint was in
+    flight at once."""
+    TASKS = 100
     lock = threading.Lock()
 
     def __init__(self, host, log):
@@ -25,21 +29,33 @@ class FakeDevice:
     def footprint(self, scheds, tuning=None):
         return 100 * len(scheds)
 
-    def count_batch(self, scheds, tuning=None):
+    def count_tasks(self, s):
+        return FakeDevice.TASKS
+
+    def count_batch_progress(self, scheds, begin, tuning=None):
         with FakeDevice.lock:
             self.log["in_flight"] += self.footprint(scheds)
             self.log["peak"] = max(self.log["peak"], self.log["in_flight"])
             self.log["sizes"].append(len(scheds))
         time.sleep(0.002 * len(scheds))
-        out = [gp.CountResult(self.value(s), self.value(s), None) for s in scheds]
+        done = [min(FakeDevice.TASKS, b + self.step(s)) for s, b in zip(scheds, begin)]
+        raws = [(d - b) * self.unit(s) * (s.redundancy or 1) for s, b, d in zip(scheds, begin, done)]
         with FakeDevice.lock:
             self.log["in_flight"] -= self.footprint(scheds)
-        return out, gp.Stats()
+        return done, raws, gp.Stats(), all(d == FakeDevice.TASKS for d in done)
+
+    @staticmethod
+    def unit(s):
+        d = s.to_dict()
+        return hash((tuple(d["adj"]), tuple(map(tuple, d["restrict_pair"])), d["size"])) & 0xFFFF
+
+    @staticmethod
+    def step(s):
+        return 40 + 30 * (FakeDevice.unit(s) % 3)  # 1, 2 or 3 slices
 
     @staticmethod
     def value(s):
-        d = s.to_dict()
-        return hash((tuple(d["adj"]), tuple(map(tuple, d["restrict_pair"])), d["size"])) & 0xFFFFFFFF
+        return FakeDevice.TASKS * FakeDevice.unit(s)
 
 
 def _server(log, **kw):
@@ -135,3 +151,30 @@ def test_max_batch_limit():
             assert False, bad
         except ValueError:
             pass
+
+
+def test_long_counts_resume_and_deadlines_fail_alone():
+    """Counts that need several time slices resume where they stopped and
+    return the same value as alone; a request whose deadline has passed when
+    its slice ends fails with RequestTimeout and the others are unaffected."""
+    from graphpi_amd.serve import RequestTimeout
+    log = {"in_flight": 0, "peak": 0, "sizes": []}
+    rng = np.random.default_rng(5)
+    reqs = [Request(i, "ab"[i % 2], random_pattern(rng, int(rng.integers(3, 7)))) for i in range(40)]
+    with _server(log, slots=2, max_batch=4, mem_budget=10000) as srv:
+        run_load(srv, reqs, poisson_arrivals(rng, 1000.0, len(reqs)))
+        for r in reqs:
+            s = srv.schedule_for(r.graph, r.adj, r.settings)
+            assert r.wait() == FakeDevice.value(s) and r.done == r.total == FakeDevice.TASKS
+            assert r.slices == -(-FakeDevice.TASKS // FakeDevice.step(s))
+        assert max(r.slices for r in reqs) > 1
+        # a deadline already passed: the request fails after its first slice
+        late = Request(100, "a", gp.PATTERNS["house"], deadline=0.0)
+        srv.submit(late)
+        ok = srv.submit(Request(101, "a", gp.PATTERNS["house"]))
+        try:
+            late.wait(10)
+            assert False, "the deadline must fail the request"
+        except RequestTimeout as e:
+            assert late.timed_out and 0 < e.done < e.total
+        assert ok.wait(10) == FakeDevice.value(ok.sched)
