@@ -172,15 +172,16 @@ def batched_bench(gp, scale=12):
 
 
 def served_bench(gp, scales=(16, 20, 22, 24), rates=(50, 200, 800), n_req=60, seed=12, deadline_s=1.0,
-                 quantum_s=0.1, use_graph=0):
+                 quantum_s=0.05, use_graph=0, sizes=(3, 7), slots=1):
     """Counting requests against resident graphs (R-MAT edge factor 16,
     seeds 5.., one per scale), random connected 3-7-vertex patterns (uniform
     size; spanning tree + Bernoulli(0.3) extra edges) on a uniformly chosen
     graph, Poisson arrivals at each rate; every request has a deadline of
     deadline_s after its arrival.
 
-    The loop: CountServer (4 slots, batches of up to 32, time slices of
-    quantum_s, admission by footprint within 85 % of the free HBM).  Serial:
+    The loop: CountServer (slots concurrent calls, batches of up to 32 new
+    requests, time slices of quantum_s, continuing requests earliest
+    deadline first, admission by footprint within 85 % of the free HBM).  Serial:
     the same requests one at a time through plain gp_count with the same
     deadline as its time limit, first come first served at the same
     arrivals.  Each request's service time alone is measured once; the
@@ -202,23 +203,30 @@ def served_bench(gp, scales=(16, 20, 22, 24), rates=(50, 200, 800), n_req=60, se
     specs = []
     for _ in range(n_req):
         g = names[int(rng.integers(0, len(names)))]
-        specs.append((g, random_pattern(rng, int(rng.integers(3, 8)))))
+        specs.append((g, random_pattern(rng, int(rng.integers(sizes[0], sizes[1] + 1)))))
     out = {"workload": "R-MAT scales " + "/".join(str(x) for x in scales) +
-                       f", edge factor 16, seeds 5-{4 + len(scales)}; random connected 3-7-vertex patterns; "
-                       f"Poisson arrivals; deadline {deadline_s:g} s after arrival",
-           "requests_per_rate": n_req, "deadline_s": deadline_s, "quantum_s": quantum_s, "rates": {}}
+                       f", edge factor 16, seeds 5-{4 + len(scales)}; random connected {sizes[0]}-{sizes[1]}-vertex "
+                       f"patterns; Poisson arrivals" +
+                       (f"; deadline {deadline_s:g} s after arrival" if deadline_s else ""),
+           "requests_per_rate": n_req, "deadline_s": deadline_s, "quantum_s": quantum_s, "slots": slots,
+           "rates": {}}
     tun = gp.Tuning()
     tun.use_graph = use_graph
-    with CountServer(graphs, slots=4, max_batch=32, tuning=tun, quantum_s=quantum_s, deadline_s=deadline_s) as srv:
+    with CountServer(graphs, slots=slots, max_batch=32, tuning=tun, quantum_s=quantum_s,
+                     deadline_s=deadline_s) as srv:
         out["resident_graph_bytes"] = sum(d.nbytes() for d in srv._dev.values())
         out["admission_budget_bytes"] = srv._budget
+        log(f"served: {len(graphs)} graphs resident ({out['resident_graph_bytes'] / 2**30:.2f} GiB), admission "
+            f"budget {srv._budget / 2**30:.1f} GiB ({time.perf_counter() - t0:.1f} s)")
         scheds = [srv.schedule_for(g, adj, (1, 1, 1)) for g, adj in specs]
         # each request alone, once: plain gp_count with the deadline as its time limit
         service, alone = [], []
         for (g, adj), s in zip(specs, scheds):
             t = gp.Tuning()
-            t.time_limit_s = deadline_s
-            t.slice_ms = max(1, int(quantum_s * 1000))
+            if deadline_s:
+                t.time_limit_s = deadline_s
+                t.abort_s = deadline_s
+                t.slice_ms = max(1, int(quantum_s * 1000))
             a = time.perf_counter()
             try:
                 r = srv._dev[g].count(s, t)
@@ -226,8 +234,10 @@ def served_bench(gp, scales=(16, 20, 22, 24), rates=(50, 200, 800), n_req=60, se
                 r = None
             service.append(time.perf_counter() - a)
             alone.append(r)
-        log(f"served: {sum(r is not None for r in alone)} of {n_req} requests finish alone within "
-            f"{deadline_s:g} s ({sum(service):.1f} s of service in all)")
+            log(f"served: alone {len(alone)}/{n_req}: {adj.shape[0]}-vertex pattern on {g}: "
+                f"{service[-1] * 1e3:.1f} ms{'' if r is not None else ' (deadline passed)'}")
+        log(f"served: {sum(r is not None for r in alone)} of {n_req} requests finish alone"
+            f"{f' within {deadline_s:g} s' if deadline_s else ''} ({sum(service):.1f} s of service in all)")
         classes = {}
         for (g, adj), r in zip(specs, alone):
             c = classes.setdefault(f"{adj.shape[0]}v {g}", {"requests": 0, "alone_finished": 0})
@@ -245,7 +255,7 @@ def served_bench(gp, scales=(16, 20, 22, 24), rates=(50, 200, 800), n_req=60, se
             clock, slat, s_ok = 0.0, [], 0
             for off, sv, a in zip(offs, service, alone):
                 start = max(off, clock)
-                budget = off + deadline_s - start
+                budget = off + deadline_s - start if deadline_s else float("inf")
                 if budget <= 0:
                     clock = start
                 elif a is not None and sv <= budget:
@@ -593,9 +603,14 @@ def main():
             roofline["balg_GBps_work_measure"] = per_pattern[dom]["balg_GBps"]
 
     batched = None if args.no_batched else batched_bench(gp, scale=args.batched_scale)
-    served = None if args.no_served else served_bench(
-        gp, scales=tuple(int(x) for x in args.served_scales.split(",")), n_req=args.served_requests,
-        deadline_s=args.served_deadline)
+    served = served_light = None
+    if not args.no_served:
+        served = served_bench(gp, scales=tuple(int(x) for x in args.served_scales.split(",")),
+                              n_req=args.served_requests, deadline_s=args.served_deadline)
+        # light requests on small graphs, no deadline: where batching new
+        # requests into shared launches pays (every request finishes)
+        served_light = served_bench(gp, scales=(8, 9, 10, 11), n_req=120, deadline_s=None, quantum_s=0.05,
+                                    sizes=(3, 6), slots=2)
 
     cpu = None
     if not args.no_cpu:
@@ -635,6 +650,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "served_light": served_light,
     }
     print(json.dumps(out), flush=True)
 

@@ -122,6 +122,8 @@ class Tuning(ctypes.Structure):
         ("no_special", ctypes.c_int32),
         ("time_limit_s", ctypes.c_double),
         ("merge_ratio", ctypes.c_int32),
+        ("abort_s", ctypes.c_double),
+        ("task_abort_s", ctypes.c_double),
     ]
 
 
@@ -194,7 +196,8 @@ _SIGS = {
                                       ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(Stats)]),
     "gp_count_batch_progress": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SchedFlat), ctypes.c_int32,
                                                ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_uint64),
-                                               ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32),
                                                ctypes.POINTER(Stats)]),
     "gp_tc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Stats)]),
     "gp_kernel_plan": (ctypes.c_int, [ctypes.POINTER(SchedFlat), ctypes.POINTER(PlanInfo)]),
@@ -487,23 +490,27 @@ class DeviceGraph:
                                     ctypes.byref(st)))
         return [CountResult(cnts[i], raws[i], None) for i in range(n)], st
 
-    def count_batch_progress(self, scheds, begin=None, tuning=None):
+    def count_batch_progress(self, scheds, begin=None, tuning=None, abort_s=None):
         """gp_count_batch_progress: count each schedule's tasks from begin[i]
-        (default 0) in bounded launches until done or tuning.time_limit_s
-        runs out.  Returns (done, raws, stats, finished): done[i] is the first
-        task not counted, raws[i] the u64 raw sum of tasks [begin[i],
-        done[i]); finished is True when every count is complete."""
+        (default 0) in bounded launches until done, tuning.time_limit_s runs
+        out or a schedule's abort time (abort_s[i], seconds after the call
+        starts) passes.  Returns (done, raws, stats, finished, aborted):
+        done[i] is the first task not counted, raws[i] the u64 raw sum of
+        tasks [begin[i], done[i]), aborted[i] whether schedule i's abort time
+        stopped it; finished is True when every count is complete."""
         n = len(scheds)
         arr = (SchedFlat * n)(*scheds)
         b = (ctypes.c_uint64 * n)(*(begin or [0] * n))
+        ab_s = None if abort_s is None else (ctypes.c_double * n)(*[x if x is not None else 0.0 for x in abort_s])
         done = (ctypes.c_uint64 * n)()
         raws = (ctypes.c_uint64 * n)()
+        ab = (ctypes.c_int32 * n)()
         st = Stats()
-        rc = lib().gp_count_batch_progress(self._h, arr, n, ctypes.byref(tuning) if tuning else None, b, done, raws,
-                                           ctypes.byref(st))
+        rc = lib().gp_count_batch_progress(self._h, arr, n, ctypes.byref(tuning) if tuning else None, b, ab_s, done,
+                                           raws, ab, ctypes.byref(st))
         if rc not in (0, GP_ETIME):
             _check(rc)
-        return list(done), list(raws), st, rc == 0
+        return list(done), list(raws), st, rc == 0, [bool(x) for x in ab]
 
     def footprint(self, scheds, tuning=None):
         """Device bytes (besides the graph) a count_batch of scheds would use."""

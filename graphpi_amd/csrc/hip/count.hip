@@ -63,13 +63,13 @@
 #define GP_EXT_NONE 0
 #define GP_EXT_SEARCH 1
 #define GP_EXT_PAIR 2
+#define GP_EXT_MERGE 3
 #define GP_SHARED_MIN_DEG 4096 // the flagged shared search only on graphs with a larger max degree
 #define GP_SAMPLE_MIN 64   // LDS entries a sampled index needs at least
 #define GP_BITMAP_RATIO 16 // a block set goes into an LDS bitmap when its words <= this x its sorted table
 #define GP_FLAG_VMAX ((1 << 23) - 1) // flagged LDS tables pack vertex ids below this with 8 flag bits
 #define GP_MERGE_MIN 256   // merge-path intersection: the shorter list has at least this many elements
 #define GP_MERGE_TILE 256  // merged elements per merge-path tile, at most (LDS: two lists of this many)
-#define GP_MERGE_RATIO 32  // default gp_tuning.merge_ratio
 #define GP_FETCH_US 1000   // adaptive fetch size: microseconds of work a fetch aims at
 #define GP_GRAPH_CACHE 8          // instantiated launch graphs kept per call context
 
@@ -103,6 +103,8 @@ struct KArgs {
     int32_t merge_ratio;  // prefix builds: merge-path intersection up to this length ratio (0: never)
     int64_t fetch_ticks;  // adaptive fetch size: wall-clock ticks a fetch aims at (0: always chunk items)
     int64_t deadline_ticks; // first launch: stop fetching after this many ticks (0: run every item)
+    unsigned long long* aborted; // per plan: set when a wave abandoned one of its tasks (KPlan::abort_ticks)
+    unsigned long long* origin;  // start time of the call's first launch (zero before it)
     unsigned long long* heavy;     // deferred: plan << 56 | task << 24 | min(candidates, 2^24 - 1)
     unsigned long long* heavy_cnt; // number deferred (may exceed heavy_cap)
     uint64_t heavy_cap;            // entries of the deferred list; the rest run in place
@@ -505,6 +507,19 @@ __device__ __forceinline__ void staged_flags_n(const int32_t* tab, int lg, const
     }
 }
 
+// a plan's abort time (from the call's first launch, origin) or its
+// per-task limit (from the wave's current task, ttask) has passed; then the
+// plan's abort flag is raised (its sums are discarded), so every caller that
+// abandons work on this answer leaves the plan flagged
+__device__ __forceinline__ bool plan_expired(const KPlan* pl, uint64_t origin, uint64_t ttask,
+                                             unsigned long long* flag) {
+    const uint64_t now = first_lane_u64(wall_clock64());
+    const bool gone = (pl->abort_ticks > 0 && now - origin > (uint64_t)pl->abort_ticks) ||
+                      (pl->task_ticks > 0 && now - ttask > (uint64_t)pl->task_ticks);
+    if (gone && threadIdx.x == 0) atomicOr(flag, 1ull);
+    return gone;
+}
+
 // FAITHFUL: replay the reference's intersections and tally B_alg.
 // PARTS: run the parts of deferred heavy tasks instead of the task list.
 // MULTI: several plans share the launch (one plan when false).
@@ -512,7 +527,8 @@ __device__ __forceinline__ void staged_flags_n(const int32_t* tab, int lg, const
 // register allocation they would otherwise burden:
 //   GP_EXT_SEARCH: the batched level's flagged shared search
 //                  (KPlan::bl_base) and sampled indexes of large sets;
-//   GP_EXT_PAIR:   the pair level grouped by v1 (KPlan::pair_v1).
+//   GP_EXT_PAIR:   the pair level grouped by v1 (KPlan::pair_v1);
+//   GP_EXT_MERGE:  merge-path prefix builds (gp_tuning.merge_ratio > 0).
 // A single-schedule launch needs at most one of them (a pair plan has a
 // plain leaf, a flagged plan an IEP leaf).
 template <bool FAITHFUL, bool PARTS, bool MULTI, int EXT>
@@ -541,7 +557,24 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     __shared__ int32_t s_bemb[GP_MAX_BATCH_BLOCKS]; // bit j: emb[j] in G (kinds 0 and 1)
     __shared__ int32_t s_off[WAVE + 1];
     __shared__ int32_t s_off2[WAVE + 1];
+    __shared__ uint64_t s_torigin; // start of the call's first launch (plans' abort times count from it)
+    __shared__ uint64_t s_ttask;   // start of the wave's current fetch (a part, in the parts launch)
     __shared__ int32_t s_plane[WAVE]; // lane of the r-th probed candidate
+
+    // A plan with an abort time (KPlan::abort_ticks, for counts that must
+    // end by a deadline even when a single task would run past it) or a
+    // per-task limit (KPlan::task_ticks, timed per fetch, so that a wave's
+    // tail after the launch's deadline stays within it): once either has
+    // passed, a wave
+    // abandons the plan's task it is in, wherever it is, skips the plan's
+    // other tasks and raises a.aborted[plan].  That plan's
+    // sums are then meaningless and the host discards them; the launch's
+    // other plans are unaffected.
+#define GP_PLAN_EXPIRED()                                                                                \
+    ((pl->abort_ticks > 0 || pl->task_ticks > 0) &&                                                     \
+     plan_expired(pl, first_lane_u64(s_torigin), first_lane_u64(s_ttask), &a.aborted[cur_plan]))
+    // (checked per task, per loop iteration and per 64 candidates of a
+    // batched or pair level)
 
     // the plan of the current work item; switching plans flushes the sums
     int cur_plan = 0;
@@ -591,7 +624,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             const int32_t fl = s_len[f];
             if (FAITHFUL) balg += 4ull * (uint64_t)((int64_t)fl + deg);
             len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0, s_hash, a.hash_slots,
-                                   FAITHFUL ? 0 : a.merge_ratio);
+                                   EXT == GP_EXT_MERGE ? a.merge_ratio : 0);
             ptr = out;
             __syncthreads(); // make the new set visible to every lane
         }
@@ -881,6 +914,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         const uint32_t leaf_mask = plain ? pl->rmask[L] : 0;
         // 2. candidates in batches of 64
         for (int32_t base = 0; base < ncand; base += WAVE) {
+            if (GP_PLAN_EXPIRED()) break;
             const int32_t ci = base + lane;
             bool valid = ci < ncand;
             const int32_t v = valid ? cand[(int64_t)ci * cstep] : -1;
@@ -1228,6 +1262,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         for (int32_t abase = 0; abase < Dn; abase += WAVE) stream(0, a_of(abase + lane));
         __threadfence();
         for (int32_t cbase = 0; cbase < Cn; cbase += WAVE) {
+            if (GP_PLAN_EXPIRED()) break;
             const int32_t ci = cbase + lane;
             bool vc = ci < Cn;
             const int32_t vv = vc ? C[ci] : -1;
@@ -1292,6 +1327,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         int32_t* cnts = s_hash + pair_slots;
         const int32_t cap_elems = pair_slots / 2;
         for (int32_t abase = 0; abase < Dn; abase += WAVE) {
+            if (GP_PLAN_EXPIRED()) break;
             const int32_t ai = abase + lane;
             bool va = ai < Dn;
             const int32_t av = va ? D[(int64_t)ai * astep] : -1;
@@ -1582,6 +1618,17 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         s_tfetch = 0;
         s_tstart = wall_clock64();
         s_chunk = a.fetch_ticks > 0 ? 1 : a.chunk;
+        // plans' hard stops count from the start of the call's first launch:
+        // its first wave to start records the time, the parts launch reads it
+        s_torigin = s_tstart;
+        s_ttask = s_tstart;
+        if (!PARTS) {
+            const unsigned long long old = atomicCAS(a.origin, 0ull, (unsigned long long)s_tstart);
+            if (old != 0) s_torigin = old;
+        } else {
+            const unsigned long long o = __hip_atomic_load(a.origin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (o != 0) s_torigin = o;
+        }
     }
     __syncthreads();
     uint64_t cur = 0, cend = 0;
@@ -1590,16 +1637,16 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         if (!PARTS) {
             bool done = false;
             while (cur >= cend) {
-                if (cur >= s_fend) {
+                if (cur >= first_lane_u64(s_fend)) {
                     if (EXT == GP_EXT_PAIR) pv_clear(); // the dense array is zero between fetches
-                    if (a.deadline_ticks > 0 && s_fend > 0 &&
-                        first_lane_u64(wall_clock64()) - s_tstart > (uint64_t)a.deadline_ticks) {
+                    if (a.deadline_ticks > 0 && first_lane_u64(s_fend) > 0 &&
+                        first_lane_u64(wall_clock64()) - first_lane_u64(s_tstart) > (uint64_t)a.deadline_ticks) {
                         done = true;
                         break;
                     }
-                    int32_t chunk_now = s_chunk;
-                    if (a.fetch_ticks > 0 && s_fend > 0) {
-                        const uint64_t el = wall_clock64() - s_tfetch;
+                    int32_t chunk_now = __builtin_amdgcn_readfirstlane(s_chunk);
+                    if (a.fetch_ticks > 0 && first_lane_u64(s_fend) > 0) {
+                        const uint64_t el = first_lane_u64(wall_clock64()) - first_lane_u64(s_tfetch);
                         const uint64_t per = max(el / (uint64_t)chunk_now, (uint64_t)1);
                         chunk_now = (int32_t)min((uint64_t)a.chunk, max((uint64_t)a.fetch_ticks / per, (uint64_t)1));
                         chunk_now = __builtin_amdgcn_readfirstlane(chunk_now);
@@ -1615,7 +1662,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     if (lane == 0) {
                         s_chunk = chunk_now;
                         s_fend = min((uint64_t)base + (uint64_t)chunk_now, n_items);
-                        if (a.fetch_ticks > 0) s_tfetch = wall_clock64();
+                        s_tfetch = wall_clock64();
+                        s_ttask = s_tfetch; // the per-task limit counts per fetch
                     }
                     __syncthreads();
                     cur = base;
@@ -1630,7 +1678,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     }
                 }
                 const uint64_t pend = a.task_off[p] + a.task_cnt[p];
-                const uint64_t fend = s_fend;
+                const uint64_t fend = first_lane_u64(s_fend);
                 if (cur < pend) {
                     cend = min(fend, pend);
                 } else { // padding after the plan's tasks: on to the next plan's items
@@ -1644,7 +1692,12 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             ++taken;
         } else {
             unsigned long long w = 0;
-            if (lane == 0) w = atomicAdd(a.counter2, 1ull);
+            __syncthreads(); // every lane is done with the last part's start time
+            if (lane == 0) {
+                w = atomicAdd(a.counter2, 1ull);
+                s_ttask = wall_clock64();
+            }
+            __syncthreads();
             w = first_lane_u64(w);
             if (w >= n_items) break;
             const unsigned long long e = a.heavy[w / GP_SPLIT_PARTS];
@@ -1662,6 +1715,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             dstride = parts;
             t = (e >> 24) & 0xFFFFFFFFull;
         }
+        if (GP_PLAN_EXPIRED()) continue; // (flagged)
         // the task: depth 0 and 1 from the edge (v0, v1), then depth 2 on
         int32_t v0, v1 = -1;
         if (pl->root_mode) {
@@ -1726,6 +1780,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         // depth-first walk over depths 2 .. L-1 (L-2 when batched)
         int d = 2;
         for (;;) {
+            if (GP_PLAN_EXPIRED()) break; // (flagged)
             const int32_t idx = s_li[d];
             if (idx >= s_ll[d]) {
                 if (d == 2) break;
@@ -1761,6 +1816,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             enter(d);
         }
     }
+#undef GP_PLAN_EXPIRED
     flush();
     if (lane == 0) a.clocks[2 * blockIdx.x + 1] = wall_clock64();
 }
@@ -1875,6 +1931,8 @@ void destroy_ctx(CallCtx* c) {
     delete c;
 }
 
+hipError_t ensure(void** p, size_t* cap, size_t bytes, hipStream_t stream);
+
 int acquire_ctx(gp_graph* g, size_t scratch_bytes, CallCtx** out) {
     CallCtx* c = nullptr;
     {
@@ -1887,8 +1945,9 @@ int acquire_ctx(gp_graph* g, size_t scratch_bytes, CallCtx** out) {
     if (!c) {
         c = new CallCtx();
         hipError_t e = hipSuccess;
-        if (e == hipSuccess) e = hipMalloc((void**)&c->d_heavy, GP_HEAVY_CAP * sizeof(unsigned long long));
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e == hipSuccess)
+            e = hipMallocAsync((void**)&c->d_heavy, GP_HEAVY_CAP * sizeof(unsigned long long), c->stream);
         if (e == hipSuccess) e = hipEventCreate(&c->e0);
         if (e == hipSuccess) e = hipEventCreate(&c->e1);
         if (e != hipSuccess) {
@@ -1897,10 +1956,8 @@ int acquire_ctx(gp_graph* g, size_t scratch_bytes, CallCtx** out) {
         }
     }
     if (c->scratch_bytes < scratch_bytes) {
-        if (c->d_scratch) (void)hipFree(c->d_scratch);
-        c->d_scratch = nullptr;
-        c->scratch_bytes = 0;
-        hipError_t e = hipMalloc((void**)&c->d_scratch, scratch_bytes);
+        size_t cap = c->scratch_bytes;
+        hipError_t e = ensure((void**)&c->d_scratch, &cap, scratch_bytes, c->stream);
         if (e != hipSuccess) {
             destroy_ctx(c);
             return gp_fail(GP_ENOMEM, std::string("scratch allocation of ") + std::to_string(scratch_bytes) +
@@ -1917,13 +1974,16 @@ void release_ctx(gp_graph* g, CallCtx* c) {
     g->idle.push_back(c);
 }
 
-// grow a device buffer to at least `bytes`
-hipError_t ensure(void** p, size_t* cap, size_t bytes) {
+// grow a device buffer to at least `bytes`, stream-ordered on the call's
+// stream: hipFree and hipMalloc would wait for the whole device, so one
+// call growing its buffers would stall every other call in flight (the
+// serving loop runs several at once)
+hipError_t ensure(void** p, size_t* cap, size_t bytes, hipStream_t stream) {
     if (*cap >= bytes) return hipSuccess;
-    if (*p) (void)hipFree(*p);
+    if (*p) (void)hipFreeAsync(*p, stream);
     *p = nullptr;
     *cap = 0;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = hipMallocAsync(p, bytes, stream);
     if (e == hipSuccess) *cap = bytes;
     return e;
 }
@@ -2121,11 +2181,13 @@ const void* count_kernel_fn(bool faithful, bool parts, bool multi, int ext) {
         if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, GP_EXT_NONE>);
         if (ext == GP_EXT_SEARCH) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_SEARCH>);
         if (ext == GP_EXT_PAIR) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_PAIR>);
+        if (ext == GP_EXT_MERGE) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_MERGE>);
         return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_NONE>);
     }
     if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, GP_EXT_NONE>);
     if (ext == GP_EXT_SEARCH) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_SEARCH>);
     if (ext == GP_EXT_PAIR) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_PAIR>);
+    if (ext == GP_EXT_MERGE) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_MERGE>);
     return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_NONE>);
 }
 
@@ -2291,7 +2353,7 @@ int ensure_support(gp_graph* g, CallCtx* c) {
     if (g->d_sup || g->e_cnt == 0) return GP_OK;
     int32_t* sup = nullptr;
     HIP_TRY(hipMalloc((void**)&sup, sizeof(int32_t) * (size_t)g->e_cnt));
-    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long), c->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(256), 0, c->stream, c->d_cnt, 8);
         e = hipGetLastError();
@@ -2341,9 +2403,9 @@ int run_house(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     CallCtx* c = nullptr;
     int rc = acquire_ctx(g, 64, &c);
     if (rc != GP_OK) return rc;
-    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long), c->stream);
     if (e == hipSuccess && dense_bytes > c->dense_bytes) {
-        e = ensure((void**)&c->d_dense, &c->dense_bytes, dense_bytes);
+        e = ensure((void**)&c->d_dense, &c->dense_bytes, dense_bytes, c->stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(zero_i32_kernel, dim3(4096), dim3(256), 0, c->stream, c->d_dense,
                                (uint64_t)(c->dense_bytes / sizeof(int32_t)));
@@ -2352,7 +2414,7 @@ int run_house(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
     }
     if (e == hipSuccess && c->clocks_cap < (size_t)waves) {
         size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
-        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves);
+        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves, c->stream);
         c->clocks_cap = e == hipSuccess ? (size_t)waves : 0;
     }
     if (e != hipSuccess) {
@@ -2434,11 +2496,16 @@ int run_house(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tuning, 
 // launch stops fetching after that long (the kernel's deadline), and
 // plan_done[p] receives how many of plan p's tasks, from its first, were
 // counted (all of them without a deadline).
+// abort_ms (optional, per plan, > 0 to apply): plan p's tasks are abandoned
+// that long after the first launch started (KPlan::abort_ticks); aborted[p]
+// then receives 1 and plan p's raw sum and progress are meaningless.
 int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tuning* tuning, uint64_t* raw,
               int64_t* count, gp_stats* stats, const uint64_t* rb = nullptr, const uint64_t* re = nullptr,
-              double deadline_ms = 0, uint64_t* plan_done = nullptr) {
+              double deadline_ms = 0, uint64_t* plan_done = nullptr, const double* abort_ms = nullptr,
+              int32_t* aborted = nullptr, double task_ms = 0) {
     // schedule-specialised kernels for single counts of shapes they cover
-    if (nplans == 1 && !rb && deadline_ms <= 0 && !(tuning && (tuning->collect_balg || tuning->no_special))) {
+    if (nplans == 1 && !rb && deadline_ms <= 0 && !abort_ms && task_ms <= 0 &&
+        !(tuning && (tuning->collect_balg || tuning->no_special))) {
         const int shape = gp_sched_shape(&scheds[0]);
         if (shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL)
             return run_house(g, scheds, tuning, shape == GP_SHAPE_HOUSE_ALL, raw, count, stats);
@@ -2451,6 +2518,12 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         if (rc != GP_OK) return rc;
     }
     HIP_TRY(hipSetDevice(g->device));
+    if (abort_ms)
+        for (int p = 0; p < nplans; ++p)
+            ln.kps[p].abort_ticks =
+                abort_ms[p] > 0 ? std::max<int64_t>(1, (int64_t)(abort_ms[p] * g->wall_khz)) : 0;
+    if (task_ms > 0)
+        for (int p = 0; p < nplans; ++p) ln.kps[p].task_ticks = std::max<int64_t>(1, (int64_t)(task_ms * g->wall_khz));
     const bool faithful = ln.faithful, any_split = ln.any_split;
     const std::vector<KPlan>& kps = ln.kps;
     const std::vector<uint64_t>&task_off = ln.task_off, &task_first = ln.task_first, &task_cnt = ln.task_cnt;
@@ -2469,12 +2542,14 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     if (rc != GP_OK) return rc;
     const size_t plans_bytes = sizeof(KPlan) * (size_t)nplans;
     const size_t meta_bytes = plans_bytes + sizeof(uint64_t) * (3 * (size_t)nplans + 1);
-    const size_t n_cnt = 3 + 3 * (size_t)nplans;
-    hipError_t e = ensure((void**)&c->d_meta, &c->meta_cap, meta_bytes);
-    if (e == hipSuccess) e = ensure((void**)&c->d_cnt, &c->cnt_cap, n_cnt * sizeof(unsigned long long));
+    // counters: fetch, parts fetch, deferred count, 3 per plan, an abort
+    // flag per plan, the first launch's start time
+    const size_t n_cnt = 4 + 4 * (size_t)nplans;
+    hipError_t e = ensure((void**)&c->d_meta, &c->meta_cap, meta_bytes, c->stream);
+    if (e == hipSuccess) e = ensure((void**)&c->d_cnt, &c->cnt_cap, n_cnt * sizeof(unsigned long long), c->stream);
     if (e == hipSuccess && dense_bytes > c->dense_bytes) {
         // a new array starts at zero; the kernels leave it at zero
-        e = ensure((void**)&c->d_dense, &c->dense_bytes, dense_bytes);
+        e = ensure((void**)&c->d_dense, &c->dense_bytes, dense_bytes, c->stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(zero_i32_kernel, dim3(4096), dim3(256), 0, c->stream, c->d_dense,
                                (uint64_t)(c->dense_bytes / sizeof(int32_t)));
@@ -2484,7 +2559,7 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     if (e == hipSuccess && c->clocks_cap < (size_t)waves) {
         // start and finish stamps per wave, for each of the two launches
         size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
-        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves);
+        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves, c->stream);
         c->clocks_cap = e == hipSuccess ? (size_t)waves : 0;
     }
     if (e != hipSuccess) {
@@ -2526,8 +2601,10 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     a.hash_lg = hash_lg;
     a.probe_pct = tuning && tuning->probe_pct > 0 ? tuning->probe_pct : 100;
     a.bitmap_ratio = !tuning || tuning->bitmap_ratio == 0 ? GP_BITMAP_RATIO : std::max(0, tuning->bitmap_ratio);
-    a.merge_ratio = !tuning || tuning->merge_ratio == 0 ? GP_MERGE_RATIO : std::max(0, tuning->merge_ratio);
+    a.merge_ratio = tuning ? std::max(0, tuning->merge_ratio) : 0;
     a.deadline_ticks = deadline_ms > 0 ? std::max<int64_t>(1, (int64_t)(deadline_ms * g->wall_khz)) : 0;
+    a.aborted = c->d_cnt + 3 + 3 * (size_t)nplans;
+    a.origin = c->d_cnt + 3 + 4 * (size_t)nplans;
     a.dense = dense_bytes ? c->d_dense : nullptr;
     a.dense_stride = dense_stride;
     a.clocks = c->d_clocks;
@@ -2536,7 +2613,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     const bool parts_launch = any_split && n_tasks > 0;
     const bool multi = nplans > 1;
     // the extended kernel for a plan that takes the flagged shared search or
-    // the pair level grouped by v1.  Multi-plan launches stay on the plain
+    // the pair level grouped by v1, or for merge-path prefix builds when the
+    // caller asks for them.  Multi-plan launches stay on the plain
     // instantiations: an extended multi-plan kernel spilled far more
     // (128 B/lane) than the shared search saved in a batch of mostly cheap
     // patterns.
@@ -2547,6 +2625,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
             ext = GP_EXT_SEARCH;
         else if (kp.pair_v1 && dense_bytes)
             ext = GP_EXT_PAIR;
+        else if (tuning && tuning->merge_ratio > 0)
+            ext = GP_EXT_MERGE;
     }
     float ms = 0;
     bool launched = false;
@@ -2596,6 +2676,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         return gp_fail((int)e, std::string("count kernel: ") + hipGetErrorString(e));
     }
     release_ctx(g, c);
+    if (aborted)
+        for (int p = 0; p < nplans; ++p) aborted[p] = host_out[3 + 3 * (size_t)nplans + p] ? 1 : 0;
     if (plan_done) {
         // the items below the counter's final value were counted
         const uint64_t upto = a.deadline_ticks > 0 ? std::min<uint64_t>(host_out[0], n_tasks) : n_tasks;
@@ -2683,10 +2765,10 @@ int run_tri_edge_tri(gp_graph* g, const gp_sched_flat* sched, uint64_t* raw, int
     rc = acquire_ctx(g, 64, &c);
     if (rc != GP_OK) return rc;
     const unsigned waves = (unsigned)(g->num_cu * 16);
-    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long));
+    hipError_t e = ensure((void**)&c->d_cnt, &c->cnt_cap, 8 * sizeof(unsigned long long), c->stream);
     if (e == hipSuccess && c->clocks_cap < (size_t)waves) { // the per-wave 128-bit sums go there
         size_t cap = c->clocks_cap * 4 * sizeof(unsigned long long);
-        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves);
+        e = ensure((void**)&c->d_clocks, &cap, sizeof(unsigned long long) * 4 * (size_t)waves, c->stream);
         c->clocks_cap = e == hipSuccess ? (size_t)waves : 0;
     }
     if (e != hipSuccess) {
@@ -2797,6 +2879,16 @@ extern "C" int gp_graph_upload(const int32_t* edge, const uint32_t* vertex, int3
     HIP_TRY(hipGetDevice(&g->device));
     HIP_TRY(hipDeviceGetAttribute(&g->num_cu, hipDeviceAttributeMultiprocessorCount, g->device));
     {
+        // call buffers come from the device's default pool (stream-ordered);
+        // keep what calls free in the pool instead of handing it back to the
+        // driver at every synchronisation, so that calls reuse it
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, g->device) == hipSuccess && pool) {
+            uint64_t keep = ~0ull;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
+    {
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) == hipSuccess && khz > 0)
             g->wall_khz = khz;
@@ -2878,13 +2970,16 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
     if (t.task_begin < 0 || (t.task_end > 0 && t.task_end < t.task_begin)) return gp_fail(GP_EINVAL, "bad task range");
     const uint64_t begin = std::min<uint64_t>((uint64_t)t.task_begin, all_tasks);
     const uint64_t end = t.task_end > 0 ? std::min<uint64_t>((uint64_t)t.task_end, all_tasks) : all_tasks;
-    // the House kernel's unit of work is a whole root, the generic kernel's an edge slot
-    const int shape = (t.collect_balg || t.no_special) ? GP_SHAPE_GENERIC : gp_sched_shape(sched);
+    // the House kernel's unit of work is a whole root, the generic kernel's an
+    // edge slot; a hard time limit takes the generic kernel, which can stop
+    // inside a task
+    const bool hard = t.abort_s > 0;
+    const int shape = (t.collect_balg || t.no_special || hard) ? GP_SHAPE_GENERIC : gp_sched_shape(sched);
     if (shape == GP_SHAPE_TRI_EDGE_TRI && t.task_begin == 0 && t.task_end <= 0)
         return run_tri_edge_tri(g, sched, raw_sum, count, stats);
     const bool per_root = shape == GP_SHAPE_HOUSE || shape == GP_SHAPE_HOUSE_ALL;
     const double t0 = wall_seconds();
-    if (t.slice_ms < 0 || end <= begin) return run_plans(g, sched, 1, &t, raw_sum, count, stats);
+    if ((t.slice_ms < 0 && !hard) || end <= begin) return run_plans(g, sched, 1, &t, raw_sum, count, stats);
     const double slice = t.slice_ms > 0 ? (double)t.slice_ms : 2000.0;
     gp_stats acc{};
     uint64_t raw = 0, cur = begin;
@@ -2928,8 +3023,22 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
             sub.task_end = (int64_t)end;
             uint64_t r = 0, did = 0;
             gp_stats st{};
-            rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st, nullptr, nullptr, slice, &did);
+            double abort_ms = 0;
+            if (hard) {
+                abort_ms = (t.abort_s - (wall_seconds() - t0)) * 1000.0;
+                if (abort_ms <= 0) {
+                    timed_out = true;
+                    break;
+                }
+            }
+            int32_t ab = 0;
+            rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st, nullptr, nullptr, t.slice_ms < 0 ? 0 : slice, &did,
+                           hard ? &abort_ms : nullptr, &ab);
             if (rc != GP_OK) return rc;
+            if (ab) { // abandoned: the launches before it stand
+                timed_out = true;
+                break;
+            }
             raw += r;
             add(st);
             cur += did;
@@ -3019,7 +3128,7 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     }
     const size_t plans_bytes = sizeof(KPlan);
     const size_t meta_bytes = plans_bytes + sizeof(uint64_t) * 4;
-    const size_t n_cnt = 6;
+    const size_t n_cnt = 8; // fetch, parts fetch, deferred count, the plan's 3, abort flag, start time
     const size_t off_cnt = align256(meta_bytes);
     const size_t off_heavy = off_cnt + align256(n_cnt * sizeof(unsigned long long));
     const size_t off_clk = off_heavy + align256(GP_NOALLOC_HEAVY * sizeof(unsigned long long));
@@ -3059,6 +3168,8 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     a.counter2 = cnt + 1;
     a.heavy_cnt = cnt + 2;
     a.out = cnt + 3;
+    a.aborted = cnt + 6;
+    a.origin = cnt + 7;
     a.heavy = reinterpret_cast<unsigned long long*>(base + off_heavy);
     a.heavy_cap = GP_NOALLOC_HEAVY;
     a.scratch = reinterpret_cast<int32_t*>(base + off_scr);
@@ -3070,7 +3181,7 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     a.hash_lg = ln.hash_lg;
     a.probe_pct = t.probe_pct > 0 ? t.probe_pct : 100;
     a.bitmap_ratio = t.bitmap_ratio == 0 ? GP_BITMAP_RATIO : std::max(0, t.bitmap_ratio);
-    a.merge_ratio = t.merge_ratio == 0 ? GP_MERGE_RATIO : std::max(0, t.merge_ratio);
+    a.merge_ratio = 0; // the plain instantiation
     a.dense = nullptr;
     a.dense_stride = 0;
     unsigned long long* clk = reinterpret_cast<unsigned long long*>(base + off_clk);
@@ -3197,8 +3308,8 @@ extern "C" int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, in
 // schedule's count is known up to a task index: the next launch, or the
 // caller, resumes from there, and raw sums of consecutive ranges add up.
 extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* scheds, int32_t n_scheds,
-                                       const gp_tuning* tuning, const uint64_t* begin, uint64_t* done,
-                                       uint64_t* raw_sums, gp_stats* stats) {
+                                       const gp_tuning* tuning, const uint64_t* begin, const double* abort_s,
+                                       uint64_t* done, uint64_t* raw_sums, int32_t* aborted, gp_stats* stats) {
     if (!gc || !scheds || !done || !raw_sums) return gp_fail(GP_EINVAL, "null argument");
     if (n_scheds < 1 || n_scheds > GP_MAX_BATCH) return gp_fail(GP_ERANGE, "schedule count out of range");
     gp_graph* g = const_cast<gp_graph*>(gc);
@@ -3209,6 +3320,8 @@ extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* 
     const double t0 = wall_seconds();
     const int n = n_scheds;
     std::vector<uint64_t> total((size_t)n), b((size_t)n);
+    std::vector<double> limit((size_t)n);     // abort time per schedule, seconds after t0 (<= 0: none)
+    std::vector<int32_t> stopped((size_t)n, 0); // 1 once the schedule's abort time has passed
     std::vector<std::vector<int> > groups(4);
     for (int i = 0; i < n; ++i) {
         KPlan kp;
@@ -3218,6 +3331,7 @@ extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* 
         b[i] = begin ? std::min(begin[i], total[i]) : 0;
         done[i] = b[i];
         raw_sums[i] = 0;
+        limit[i] = abort_s ? abort_s[i] : t.abort_s;
         groups[kp.pair ? 0 : kp.batched ? (kp.nblocks <= 3 ? 1 : 2) : 3].push_back(i);
     }
     gp_stats acc{};
@@ -3225,21 +3339,31 @@ extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* 
     const double slice = t.slice_ms > 0 ? (double)t.slice_ms : 2000.0;
     for (const std::vector<int>& grp : groups) {
         for (;;) {
-            // the class's schedules that still have tasks, in order
+            // the class's schedules that still have tasks and time, in order
+            const double now = wall_seconds() - t0;
             std::vector<gp_sched_flat> sub;
             std::vector<uint64_t> rb, re;
+            std::vector<double> ab_ms;
             std::vector<int> who;
+            bool any_limit = false;
             for (int i : grp) {
-                if (done[i] >= total[i]) continue;
+                if (done[i] >= total[i] || stopped[i]) continue;
+                if (limit[i] > 0 && now >= limit[i]) {
+                    stopped[i] = 1;
+                    continue;
+                }
                 who.push_back(i);
                 sub.push_back(scheds[i]);
                 rb.push_back(done[i]);
                 re.push_back(total[i]);
+                // from the start of the launch about to begin
+                ab_ms.push_back(limit[i] > 0 ? (limit[i] - now) * 1000.0 : 0.0);
+                any_limit = any_limit || limit[i] > 0;
             }
             if (who.empty()) break;
             double left_ms = slice;
             if (t.time_limit_s > 0) {
-                left_ms = (t.time_limit_s - (wall_seconds() - t0)) * 1000.0;
+                left_ms = (t.time_limit_s - now) * 1000.0;
                 if (left_ms <= 0) {
                     timed_out = true;
                     break;
@@ -3247,11 +3371,17 @@ extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* 
                 left_ms = std::min(left_ms, slice);
             }
             std::vector<uint64_t> raw(sub.size()), did(sub.size());
+            std::vector<int32_t> ab(sub.size(), 0);
             gp_stats st{};
             int rc = run_plans(g, sub.data(), (int)sub.size(), &t, raw.data(), nullptr, &st, rb.data(), re.data(),
-                               left_ms, did.data());
+                               left_ms, did.data(), any_limit ? ab_ms.data() : nullptr, ab.data(),
+                               t.task_abort_s > 0 ? t.task_abort_s * 1000.0 : 0.0);
             if (rc != GP_OK) return rc;
             for (size_t k = 0; k < who.size(); ++k) {
+                if (ab[k]) { // abandoned: its launches before this one stand
+                    stopped[who[k]] = 1;
+                    continue;
+                }
                 raw_sums[who[k]] += raw[k];
                 done[who[k]] += did[k];
             }
@@ -3272,12 +3402,12 @@ extern "C" int gp_count_batch_progress(const gp_graph* gc, const gp_sched_flat* 
     for (int i = 0; i < n; ++i) {
         acc.tasks_total += total[i] - b[i];
         all_done = all_done && done[i] == total[i];
+        if (aborted) aborted[i] = stopped[i];
     }
     acc.finished = all_done ? 1 : 0;
     if (stats) *stats = acc;
     if (!all_done)
-        return gp_fail(GP_ETIME, std::string("time limit of ") + std::to_string(t.time_limit_s) +
-                                     " s reached before every schedule's count was complete");
+        return gp_fail(GP_ETIME, "the time limit or an abort time came before every schedule's count was complete");
     return GP_OK;
 }
 

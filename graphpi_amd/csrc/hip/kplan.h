@@ -78,6 +78,14 @@ struct KPlan {
     // set by the launcher: a task whose depth-2 loop has more candidates is
     // deferred and run in parts (INT32_MAX: never)
     int32_t split_min;
+    // set by the launcher: the plan's tasks are abandoned once this many
+    // wall-clock ticks have passed since the call's first launch started
+    // (0: never)
+    int64_t abort_ticks;
+    // set by the launcher: the plan is abandoned the same way when a fetch of
+    // its tasks (a part, in the parts launch) has run this many ticks
+    // (0: never)
+    int64_t task_ticks;
 };
 
 // Loop-nest shapes with a schedule-specialised kernel.

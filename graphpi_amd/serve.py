@@ -12,14 +12,15 @@ memory are free and its kernels overlap with theirs.
 
 Counts run in time slices: a batch call stops after about quantum_s (its
 launches stop fetching at a deadline, so each has counted a prefix of every
-request's tasks), finished requests are answered, and the others go back to
-the queue with their progress and continue in a later batch from the task
-where they stopped.  A long count therefore never holds a slot for long and
-short requests that arrive behind it are not stuck.  A request still
-unfinished when its deadline (seconds after arrival) has passed fails with
-RequestTimeout.  Every answered result is the exact count: the raw sums of
-consecutive task ranges add up, so slicing, batching and concurrency change
-when a request runs, never what it returns.
+request's tasks), finished requests are answered, and the others continue in
+later calls from the task where they stopped, shortest estimated remaining
+time first (CountServer explains the two queues).  A long count therefore
+never holds the device for long and short requests that arrive behind it
+are not stuck.  A request that cannot finish before its deadline (seconds
+after arrival) fails with RequestTimeout, as soon as that is known.  Every
+answered result is the exact count: the raw sums of consecutive task ranges
+add up, so slicing, batching and concurrency change when a request runs,
+never what it returns.
 
 Schedules are built on the host (the planner) and cached per (graph, pattern,
 settings), as a server would cache them per canonical pattern.
@@ -30,7 +31,7 @@ import time
 
 import numpy as np
 
-from . import DeviceGraph, build_schedule, device_memory
+from . import DeviceGraph, build_schedule, device_memory, kernel_plan
 
 
 class RequestTimeout(TimeoutError):
@@ -45,7 +46,7 @@ class RequestTimeout(TimeoutError):
 
 class Request:
     __slots__ = ("rid", "graph", "adj", "settings", "deadline", "t_arrival", "t_start", "t_done", "count", "raw",
-                 "error", "sched", "total", "done", "raw_acc", "slices", "_event")
+                 "error", "sched", "total", "done", "raw_acc", "slices", "heavy", "gpu_s", "cls", "_event")
 
     def __init__(self, rid, graph, adj, settings=(1, 1, 1), t_arrival=None, deadline=None):
         self.rid = rid
@@ -62,6 +63,9 @@ class Request:
         self.done = 0       # tasks counted so far (a prefix)
         self.raw_acc = 0    # u64 raw sum of those tasks
         self.slices = 0     # batch calls the request took part in
+        self.heavy = False  # a task ran past the probe time in a new requests' batch
+        self.gpu_s = 0.0    # device time spent on it (a call's time shared by its requests)
+        self.cls = 0        # launch class of its kernel plan (launch_class)
         self._event = threading.Event()
 
     @property
@@ -80,22 +84,47 @@ class Request:
         return None if self.t_done is None else self.t_done - self.t_arrival
 
 
+def launch_class(sched):
+    """The class of LDS size a batch call launches the schedule's plan in
+    (pair level / batched level with at most 3 blocks / more / neither), as
+    gp_count_batch and gp_count_batch_progress group plans."""
+    p = kernel_plan(sched)
+    return 0 if p["pair"] else (1 if p["blocks"] <= 3 else 2) if p["batched"] else 3
+
+
 class CountServer:
-    """graphs: {name: HostGraph}.  slots: concurrent batches on the device.
-    max_batch: schedules per launch.  mem_budget: device bytes the in-flight
-    batches may use besides the graphs (a batch larger than the budget still
-    runs, alone).  quantum_s: length of a batch call's time slice (None: a
-    batch runs until every count in it is complete).  deadline_s: default
-    deadline of a request, in seconds after its arrival (None: none)."""
+    """graphs: {name: HostGraph}.  slots: concurrent batch calls on the
+    device (slot 0 serves new requests first, the others continuing ones
+    first).  max_batch: schedules per call.  mem_budget: device bytes the
+    in-flight calls may use besides the graphs (a call larger than the budget
+    still runs, alone).  quantum_s: a call's time slice.  deadline_s: default
+    deadline of a request, in seconds after its arrival (None: none).
+    probe_s: longest a single task may run in a new requests' batch (default
+    2 x quantum).
+
+    Scheduling.  New requests of one graph are counted together in a first
+    slice with the per-task probe: most small counts finish there.  A request
+    still unfinished (its slice ran out, or one of its tasks ran past the
+    probe, which abandons it for that call without touching the others)
+    joins the continuing queue with its progress.  That queue runs shortest
+    estimated remaining time first (remaining tasks x the time per task seen
+    so far), one request per call with its deadline as the hard stop, and a
+    request whose estimate exceeds the time left before its deadline is
+    failed at once instead of holding the device until it runs out."""
 
     MAX_BATCH = 256  # schedules gp_count_batch takes per call
 
-    def __init__(self, graphs, slots=4, max_batch=32, mem_budget=None, tuning=None, device_factory=None,
-                 quantum_s=0.1, deadline_s=None):
+    def __init__(self, graphs, slots=2, max_batch=32, mem_budget=None, tuning=None, device_factory=None,
+                 quantum_s=0.05, deadline_s=None, probe_s=None, order="edf"):
         if not 1 <= max_batch <= self.MAX_BATCH:
             raise ValueError(f"max_batch must be 1..{self.MAX_BATCH}")
+        if order not in ("edf", "srpt"):
+            raise ValueError("order must be 'edf' or 'srpt'")
+        self._order = order
         self._quantum = quantum_s
         self._deadline = deadline_s
+        self._probe = probe_s if probe_s is not None else (2 * quantum_s if quantum_s else None)
+        self._slots = slots
         make = device_factory or DeviceGraph
         self._dev = {}
         self._host = {}
@@ -109,6 +138,13 @@ class CountServer:
         # gave the same counts but higher median latency on these small
         # requests (measured: p50 at 200 q/s 8-10 ms against 2.3-2.6 ms)
         self._tuning = tuning
+        # continuing requests: the time slice, the deadline as the hard stop
+        self._slice_tuning = self._tuning_for_slice()
+        # new requests: also the per-fetch probe (the parts of deferred heavy
+        # tasks run in a second launch, fetch by fetch, under the same probe)
+        self._new_tuning = self._tuning_for_slice()
+        if self._probe:
+            self._new_tuning.task_abort_s = self._probe
         self._max_batch = max_batch
         if mem_budget is None:
             # most of what is free once the graphs are resident (the rest is
@@ -117,12 +153,13 @@ class CountServer:
             mem_budget = int(free * 0.85)
         self._budget = mem_budget
         self._in_flight = 0
-        self._pending = collections.deque()
+        self._pending = collections.deque()  # new requests, arrival order
+        self._long = []                      # continuing requests
         self._cv = threading.Condition()
         self._closed = False
         self._sched_cache = {}
-        self.batches = []  # (graph, size, footprint, seconds) per launch
-        self._workers = [threading.Thread(target=self._run, name=f"count-slot-{i}", daemon=True)
+        self.batches = []  # (graph, size, footprint, seconds, kernel ms, launches) per batch call
+        self._workers = [threading.Thread(target=self._run, args=(i,), name=f"count-slot-{i}", daemon=True)
                          for i in range(slots)]
         for w in self._workers:
             w.start()
@@ -136,6 +173,7 @@ class CountServer:
         try:
             req.sched = self.schedule_for(req.graph, req.adj, req.settings)
             req.total = self._dev[req.graph].count_tasks(req.sched)
+            req.cls = launch_class(req.sched)
             if req.deadline is None:
                 req.deadline = self._deadline
         except Exception as e:
@@ -147,7 +185,7 @@ class CountServer:
             if self._closed:
                 raise RuntimeError("server closed")
             self._pending.append(req)
-            self._cv.notify()
+            self._cv.notify_all()
         return req
 
     def close(self):
@@ -178,88 +216,157 @@ class CountServer:
                 self._sched_cache[key] = s
         return s
 
-    def _take_batch(self):
-        """Called with the lock held: the requests of the oldest request's graph,
-        up to max_batch, within the memory budget.  None when nothing fits."""
-        if not self._pending:
-            return None
-        graph = self._pending[0].graph
-        batch, rest = [], collections.deque()
-        while self._pending:
-            r = self._pending.popleft()
-            if r.graph == graph and len(batch) < self._max_batch:
-                batch.append(r)
-            else:
-                rest.append(r)
-        self._pending = rest
-        # requests not yet started go first: a batch call lays the counts'
-        # remaining tasks end to end in this order, so within its time slice
-        # the new (mostly short) requests finish before resumed long ones
-        ok = sorted(batch, key=lambda r: (r.done > 0, r.t_arrival))
-        scheds = [r.sched for r in ok]
-        dev = self._dev[graph]
-        try:
-            fp = dev.footprint(scheds, self._tuning)
-            # admit the longest prefix that fits the free budget (at least one
-            # request when nothing else is in flight)
-            while ok and self._in_flight + fp > self._budget and (self._in_flight > 0 or len(ok) > 1):
-                self._pending.appendleft(ok.pop())
-                scheds.pop()
-                fp = dev.footprint(scheds, self._tuning) if scheds else 0
-        except Exception as e:
-            # the requests taken out of the queue fail; the slot keeps serving
-            now = time.perf_counter()
-            for r in ok:
-                r.error = e
-                r.t_done = now
-                r._event.set()
-            return None
-        if not ok:
-            return None
-        self._in_flight += fp
-        return (graph, ok, scheds, fp)
-
     def _tuning_for_slice(self):
         from . import Tuning
         t = Tuning()
         if self._tuning is not None:
             import ctypes
             ctypes.memmove(ctypes.byref(t), ctypes.byref(self._tuning), ctypes.sizeof(Tuning))
+        if t.waves_per_cu <= 0:
+            # the slots' launches run side by side: each gets its share of the
+            # persistent waves (and of the per-wave scratch they need)
+            t.waves_per_cu = max(4, 24 // max(1, self._slots))
         if self._quantum:
             t.time_limit_s = self._quantum
             t.slice_ms = max(1, int(self._quantum * 1000.0))
         return t
 
-    def _run(self):
-        tun = self._tuning_for_slice()
+    def _order_key(self, r, now):
+        """Continuing requests with a finite estimate (they have counted some
+        tasks) come before those with none (a task of theirs ran past the
+        probe); within each, earliest deadline first (order = "edf", the
+        default) or shortest estimated remaining time first ("srpt")."""
+        est = self._estimate(r, now)
+        if self._order == "srpt":
+            return (est == float("inf"), est)
+        return (est == float("inf"), r.t_arrival + r.deadline if r.deadline is not None else float("inf"))
+
+    @staticmethod
+    def _estimate(r, now):
+        """Seconds of device time the request still needs, from its own rate
+        so far (infinite when it has counted nothing yet)."""
+        if r.done <= 0:
+            return float("inf")
+        return (r.total - r.done) * r.gpu_s / r.done
+
+    def _fail_timeout(self, r, now):
+        r.error = RequestTimeout(r.rid, r.done, r.total)
+        r.t_done = now
+        r._event.set()
+
+    def _take_new(self):
+        """Called with the lock held: new requests of the oldest new request's
+        graph and launch class, up to max_batch.  (A batch call runs one
+        launch per class of LDS size in turn; with one class per call its
+        whole time slice goes to the one launch, and no request waits behind
+        another class's launch for a slice that never reaches it.)"""
+        graph, cls = self._pending[0].graph, self._pending[0].cls
+        batch, rest = [], collections.deque()
+        while self._pending:
+            r = self._pending.popleft()
+            if r.graph == graph and r.cls == cls and len(batch) < self._max_batch:
+                batch.append(r)
+            else:
+                rest.append(r)
+        self._pending = rest
+        return batch
+
+    def _take_long(self, now):
+        """Called with the lock held: the continuing request with the least
+        estimated remaining time; requests whose estimate exceeds the time
+        left before their deadline fail here."""
+        keep = []
+        for r in self._long:
+            left = r.t_arrival + r.deadline - now if r.deadline is not None else float("inf")
+            if left <= 0 or (r.done > 0 and self._estimate(r, now) > left):
+                self._fail_timeout(r, now)
+            else:
+                keep.append(r)
+        self._long = keep
+        if not keep:
+            return []
+        best = min(keep, key=lambda r: (self._order_key(r, now), r.t_arrival))
+        keep.remove(best)
+        return [best]
+
+    def _take_batch(self, slot):
+        """Called with the lock held: (graph, requests, schedules, footprint,
+        new) or None when nothing fits.  Slot 0 takes new requests first, the
+        others continuing ones first."""
+        now = time.perf_counter()
+        new_first = slot == 0 or not self._long
+        batch, new = [], False
+        if new_first and self._pending:
+            batch, new = self._take_new(), True
+        if not batch and self._long:
+            batch = self._take_long(now)
+        if not batch and self._pending:
+            batch, new = self._take_new(), True
+        if not batch:
+            return None
+        graph = batch[0].graph
+        scheds = [r.sched for r in batch]
+        dev = self._dev[graph]
+        tun = self._new_tuning if new else self._slice_tuning
+        try:
+            fp = dev.footprint(scheds, tun)
+            # admit the longest prefix that fits the free budget (at least one
+            # request when nothing else is in flight)
+            while batch and self._in_flight + fp > self._budget and (self._in_flight > 0 or len(batch) > 1):
+                r = batch.pop()
+                (self._pending.appendleft if new else self._long.append)(r)
+                scheds.pop()
+                fp = dev.footprint(scheds, tun) if scheds else 0
+        except Exception as e:
+            # the requests taken out of the queue fail; the slot keeps serving
+            for r in batch:
+                r.error = e
+                r.t_done = now
+                r._event.set()
+            return None
+        if not batch:
+            return None
+        self._in_flight += fp
+        return (graph, batch, scheds, fp, new)
+
+    def _run(self, slot):
         while True:
             with self._cv:
                 item = None
                 while item is None:
-                    if self._closed and not self._pending:
+                    if self._closed and not self._pending and not self._long:
                         return
-                    item = self._take_batch()
+                    item = self._take_batch(slot)
                     if item is None:
-                        self._cv.wait(0.05)
-            graph, reqs, scheds, fp = item
+                        self._cv.wait(0.01)
+            graph, reqs, scheds, fp, new = item
             t0 = time.perf_counter()
             for r in reqs:
                 if r.t_start is None:
                     r.t_start = t0
+            # each request's deadline is its count's abort time: a launch still
+            # on its tasks then abandons them, its launches before stand and
+            # the batch's other requests carry on
+            abort = [max(r.t_arrival + r.deadline - t0, 1e-3) if r.deadline is not None else 0.0 for r in reqs]
+            tun = self._new_tuning if new else self._slice_tuning
             try:
-                done, raws, _, _ = self._dev[graph].count_batch_progress(scheds, [r.done for r in reqs], tun)
+                done, raws, st, _, aborted = self._dev[graph].count_batch_progress(scheds, [r.done for r in reqs],
+                                                                                   tun, abort)
                 err = None
             except Exception as e:
-                done = raws = None
+                done = raws = aborted = st = None
                 err = e
             t1 = time.perf_counter()
-            finished, again = [], []
+            finished, again, unstarted = [], [], []
             for i, r in enumerate(reqs):
                 r.slices += 1
                 if err is not None:
                     r.error = err
                     finished.append(r)
                     continue
+                progressed = done[i] > r.done
+                if progressed or aborted[i]:
+                    r.gpu_s += (t1 - t0) / len(reqs)
                 r.raw_acc = (r.raw_acc + raws[i]) % (1 << 64)
                 r.done = done[i]
                 if r.done >= r.total:
@@ -272,13 +379,19 @@ class CountServer:
                 elif r.deadline is not None and t1 - r.t_arrival > r.deadline:
                     r.error = RequestTimeout(r.rid, r.done, r.total)
                     finished.append(r)
+                elif new and not progressed and not aborted[i]:
+                    # the slice ended before reaching its tasks: still new
+                    unstarted.append(r)
                 else:
+                    if aborted[i]:
+                        r.heavy = True
                     again.append(r)
             with self._cv:
                 self._in_flight -= fp
-                self.batches.append((graph, len(reqs), fp, t1 - t0))
-                # unfinished counts continue in a later batch, behind what waits now
-                self._pending.extend(again)
+                self.batches.append((graph, len(reqs), fp, t1 - t0, st.kernel_ms if st is not None else 0.0,
+                                     st.launches if st is not None else 0))
+                self._long.extend(again)
+                self._pending.extendleft(reversed(unstarted))
                 self._cv.notify_all()
             for r in finished:
                 r.t_done = t1

@@ -136,11 +136,26 @@ typedef struct gp_tuning {
                                time has passed and return GP_ETIME (the reference's
                                Graph::max_running_time, include/graph.h:12); every launch
                                still runs to its end.  0 = no limit */
-    int32_t merge_ratio;    /* prefix-set builds: when the longer list does not fit the LDS pool,
-                               the shorter has at least 256 elements and the longer is at most
-                               this many times as long, the two are merged (merge path, tiles
-                               staged in LDS) instead of searching the longer one per element;
-                               0 = default (32), < 0 = never.  Counts are identical either way */
+    int32_t merge_ratio;    /* > 0: prefix-set builds merge two lists (merge path, tiles staged in
+                               LDS) instead of searching the longer one per element of the shorter,
+                               when the longer does not fit the LDS pool, the shorter has at least
+                               256 elements and the longer is at most this many times as long.
+                               Single-schedule counts whose plan takes no other extended path;
+                               its own kernel instantiation.  0 = off (the default: on the R-MAT
+                               and Chung-Lu configurations restriction bounds cut hub lists short
+                               and merging measured no faster).  Counts are identical either way */
+    double abort_s;         /* gp_count, gp_count_batch_progress: when > 0, a launch still running
+                               this long after the call started abandons its work (wherever its
+                               waves are), and the call returns GP_ETIME with the result of the
+                               launches that completed before it (a prefix of the tasks).  For
+                               callers that must answer by a deadline even when one task would run
+                               past it (a serving loop).  Such a gp_count takes the generic kernel */
+    double task_abort_s;    /* gp_count_batch_progress: when > 0, a schedule one of whose work
+                               fetches (a few tasks, sized for about a millisecond from the wave's
+                               last fetch) has run this long is abandoned as at its abort time
+                               (aborted[i] = 1; its launches before stand).  Bounds how long a
+                               launch runs past its slice, and lets a caller find the counts whose
+                               tasks are long and give them launches of their own */
 } gp_tuning;
 
 /* Per-call statistics (all optional outputs). */
@@ -224,23 +239,28 @@ GP_API int gp_count_batch(const gp_graph* g, const gp_sched_flat* scheds, int32_
                           const gp_tuning* tuning, uint64_t* raw_sums, int64_t* counts, gp_stats* stats);
 
 /* gp_count_batch over part of each schedule's tasks, resumable.  Schedule i
- * counts its tasks from begin[i] (NULL: from 0) as bounded launches over the
- * schedules' remaining tasks laid end to end (gp_tuning.slice_ms sizes them;
- * with gp_tuning.time_limit_s > 0 the first launch of each LDS class takes
- * one task per persistent wave).  Between launches the time limit is
- * checked; every launch runs to its end.  On return done[i] is the first
- * task of schedule i not counted (gp_count_tasks when its count is
- * complete) and raw_sums[i] the u64 raw sum of its tasks [begin[i], done[i]).
- * Raw sums of consecutive task ranges add up, so a caller resumes a count by
- * passing done[] back as begin[] and adding the raw sums; the count is then
- * (int64)raw / redundancy, as gp_count returns it.  Returns GP_OK when every
- * schedule's count is complete, GP_ETIME when the limit stopped it first
- * (done[] and raw_sums[] are valid either way).  gp_tuning task ranges and
- * collect_balg are rejected (GP_EINVAL).  No reference counterpart: it lets
- * a serving loop give long counts time slices (graphpi_amd/serve.py). */
+ * counts its tasks from begin[i] (NULL: from 0) in launches over the
+ * schedules' remaining tasks laid end to end, each of which stops fetching
+ * after gp_tuning.slice_ms (or what is left of gp_tuning.time_limit_s) and
+ * so counts a prefix of them; the time limit is checked between launches.
+ * abort_s[i] (NULL: gp_tuning.abort_s for every schedule; <= 0: none) is
+ * schedule i's hard stop, in seconds after the call starts: a launch still
+ * on its tasks then abandons them (wherever its waves are) and that launch's
+ * part of schedule i is discarded; aborted[i] (optional) receives 1 and the
+ * schedule takes no further launches, while the others carry on.  On return
+ * done[i] is the first task of schedule i not counted (gp_count_tasks when
+ * its count is complete) and raw_sums[i] the u64 raw sum of its tasks
+ * [begin[i], done[i]).  Raw sums of consecutive task ranges add up, so a
+ * caller resumes a count by passing done[] back as begin[] and adding the
+ * raw sums; the count is then (int64)raw / redundancy, as gp_count returns
+ * it.  Returns GP_OK when every schedule's count is complete, GP_ETIME
+ * otherwise (done[] and raw_sums[] are valid either way).  gp_tuning task
+ * ranges and collect_balg are rejected (GP_EINVAL).  No reference
+ * counterpart: it lets a serving loop give long counts time slices and
+ * deadlines (graphpi_amd/serve.py). */
 GP_API int gp_count_batch_progress(const gp_graph* g, const gp_sched_flat* scheds, int32_t n_scheds,
-                                   const gp_tuning* tuning, const uint64_t* begin, uint64_t* done,
-                                   uint64_t* raw_sums, gp_stats* stats);
+                                   const gp_tuning* tuning, const uint64_t* begin, const double* abort_s,
+                                   uint64_t* done, uint64_t* raw_sums, int32_t* aborted, gp_stats* stats);
 
 /* Device memory (bytes, besides the uploaded graph) that gp_count_batch with
  * the same arguments would use: per-wave scratch, plans, counters, stamps

@@ -238,7 +238,7 @@ def test_progress_batches_resume_exactly():
     t.time_limit_s = 0.005
     begin, raws, calls = [0] * len(scheds), [0] * len(scheds), 0
     while True:
-        done, raw, st, finished = dev.count_batch_progress(scheds, begin, t)
+        done, raw, st, finished, _ = dev.count_batch_progress(scheds, begin, t)
         calls += 1
         assert all(b <= d <= tot for b, d, tot in zip(begin, done, totals))
         if calls == 1:
@@ -261,5 +261,68 @@ def test_progress_batches_resume_exactly():
     for i, s in enumerate(scheds):
         assert raws[i] == want[i].raw, names[i]
     # no time limit: one call finishes everything
-    done, raw, st, finished = dev.count_batch_progress(scheds)
-    assert finished and done == totals and raw == [w.raw for w in want]
+    done, raw, st, finished, aborted = dev.count_batch_progress(scheds)
+    assert finished and done == totals and raw == [w.raw for w in want] and not any(aborted)
+    # a schedule past its abort time is abandoned without touching the
+    # others: a 6-vertex motif whose count takes minutes here, stopped after
+    # 0.2 s, between counts that finish
+    heavy, _ = gp.build_schedule(gp.motifs(6)[77], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+    mix = [scheds[1], heavy, scheds[4]]
+    done, raw, st, finished, aborted = dev.count_batch_progress(mix, abort_s=[0.0, 0.2, 0.0])
+    assert not finished and aborted == [False, True, False]
+    assert done[0] == totals[1] and done[2] == totals[4] and done[1] < dev.count_tasks(heavy)
+    assert (raw[0], raw[2]) == (want[1].raw, want[4].raw)
+    # the same through a per-task limit: a limit of 0.1 ms abandons the
+    # motif (whose hub tasks run far longer); the schedules it leaves are
+    # exact, and a limit longer than the other counts take whole leaves them
+    t = gp.Tuning()
+    t.task_abort_s = 1e-4
+    done, raw, st, finished, aborted = dev.count_batch_progress(mix, tuning=t)
+    assert not finished and aborted[1]
+    for k, i in ((0, 1), (2, 4)):
+        if not aborted[k]:
+            assert done[k] == totals[i] and raw[k] == want[i].raw
+    t.task_abort_s = 5.0
+    done, raw, st, finished, aborted = dev.count_batch_progress([scheds[1], scheds[4]], tuning=t)
+    assert finished and not any(aborted) and raw == [want[1].raw, want[4].raw]
+
+
+def test_hard_limit_abandons_a_long_launch():
+    """gp_tuning.abort_s: a launch still running at the limit abandons its
+    work wherever its waves are, and gp_count returns GP_ETIME soon after,
+    with the result of the launches that completed before it (a prefix of
+    the tasks)."""
+    import time
+    host = gp.HostGraph.rmat(16, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = dev.triangle_counting()
+    # a 6-vertex motif whose full count here takes minutes, as one launch
+    # (and single tasks that run longer than the limit); the 5-clique (about
+    # 240 ms here) in launches of about 5 ms
+    heavy, _ = gp.build_schedule(gp.motifs(6)[77], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+    five, _ = gp.build_schedule(gp.PATTERNS["5clique"], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+    for s, slice_ms, limit in ((heavy, -1, 0.3), (five, 5, 0.1)):
+        t = gp.Tuning()
+        t.slice_ms = slice_ms
+        t.abort_s = limit
+        t0 = time.perf_counter()
+        with pytest.raises(gp.GPTimeout) as e:
+            dev.count(s, t)
+        wall = time.perf_counter() - t0
+        part = e.value.partial
+        print(f"slice_ms {slice_ms}: stopped after {wall:.3f} s, {part.stats.launches} launches completed, "
+              f"{part.stats.tasks} tasks", flush=True)
+        assert wall < 2.0
+        if slice_ms < 0:
+            assert part.stats.tasks == 0 and part.raw == 0
+        else:
+            assert 0 < part.stats.tasks < part.stats.tasks_total
+            r = gp.Tuning()
+            r.task_end = int(part.stats.tasks)
+            r.slice_ms = -1
+            assert dev.count(s, r).raw == part.raw
+    # a limit that is not reached changes nothing
+    t = gp.Tuning()
+    t.abort_s = 600.0
+    s4, _ = gp.build_schedule(gp.PATTERNS["4clique"], 1, 1, True, host.v_cnt, host.e_cnt, tri)
+    assert dev.count(s4, t).raw == dev.count(s4).raw

@@ -65,14 +65,17 @@ def test_rmat22_shape_and_triangles(rmat22):
 def test_rmat22_4clique_whole_count(rmat22):
     host, dev, tri = rmat22
     s = _sched(host, "4clique", tri)
-    got = dev.count(s)                       # bounded launches over task ranges
+    sliced = gp.Tuning()
+    sliced.slice_ms = 50                     # launches that stop fetching after 50 ms, each resuming the last
+    got = dev.count(s, sliced)
     one = gp.Tuning()
     one.slice_ms = -1
     single = dev.count(s, one)               # one launch
+    default = dev.count(s)                   # 2 s slices: one launch for this count
     print(f"4-clique device {got.count} in {got.stats.kernel_ms:.1f} ms over {got.stats.launches} launches "
           f"(one launch {single.stats.kernel_ms:.1f} ms)", flush=True)
     assert got.stats.launches > 1 and single.stats.launches == 1
-    assert (got.raw, got.count) == (single.raw, single.count)
+    assert (got.raw, got.count) == (single.raw, single.count) == (default.raw, default.count)
     # the oracle's whole count as 16 interleaved root slices (roots i, i + 16,
     # ...), whose raw sums add up; each slice reports when it is done
     t0 = time.time()

@@ -3,6 +3,7 @@
 deterministic function of the schedule), and the synthetic load generators."""
 import threading
 import time
+import zlib
 
 import numpy as np
 
@@ -32,22 +33,33 @@ class FakeDevice:
     def count_tasks(self, s):
         return FakeDevice.TASKS
 
-    def count_batch_progress(self, scheds, begin, tuning=None):
+    def count_batch_progress(self, scheds, begin, tuning=None, abort_s=None):
         with FakeDevice.lock:
             self.log["in_flight"] += self.footprint(scheds)
             self.log["peak"] = max(self.log["peak"], self.log["in_flight"])
             self.log["sizes"].append(len(scheds))
         time.sleep(0.002 * len(scheds))
-        done = [min(FakeDevice.TASKS, b + self.step(s)) for s, b in zip(scheds, begin)]
+        # a "heavy" schedule's task outlasts the per-task limit a light batch
+        # runs with (the server's probe): abandoned without progress there
+        probe = tuning is not None and tuning.task_abort_s > 0
+        aborted = [probe and self.heavy(s) for s in scheds]
+        done = [b if ab else min(FakeDevice.TASKS, b + self.step(s)) for s, b, ab in zip(scheds, begin, aborted)]
         raws = [(d - b) * self.unit(s) * (s.redundancy or 1) for s, b, d in zip(scheds, begin, done)]
         with FakeDevice.lock:
             self.log["in_flight"] -= self.footprint(scheds)
-        return done, raws, gp.Stats(), all(d == FakeDevice.TASKS for d in done)
+            if len(scheds) == 1 and self.heavy(scheds[0]):
+                self.log.setdefault("heavy_alone", 0)
+                self.log["heavy_alone"] += 1
+        return done, raws, gp.Stats(), all(d == FakeDevice.TASKS for d in done), aborted
 
     @staticmethod
     def unit(s):
         d = s.to_dict()
-        return hash((tuple(d["adj"]), tuple(map(tuple, d["restrict_pair"])), d["size"])) & 0xFFFF
+        return zlib.crc32(repr((d["adj"], d["restrict_pair"], d["size"])).encode()) & 0xFFFF
+
+    @staticmethod
+    def heavy(s):
+        return FakeDevice.unit(s) % 4 == 0
 
     @staticmethod
     def step(s):
@@ -97,7 +109,7 @@ def test_results_same_alone_and_in_the_mix():
             assert r.latency >= 0
     assert max(log["sizes"]) <= 8
     assert log["peak"] <= 1500
-    assert len(log["sizes"]) < len(reqs)  # requests were batched
+    assert max(log["sizes"]) > 1  # requests were batched
     lat = [r.latency for r in reqs]
     assert percentile(lat, 50) <= percentile(lat, 99)
 
@@ -166,12 +178,19 @@ def test_long_counts_resume_and_deadlines_fail_alone():
         for r in reqs:
             s = srv.schedule_for(r.graph, r.adj, r.settings)
             assert r.wait() == FakeDevice.value(s) and r.done == r.total == FakeDevice.TASKS
-            assert r.slices == -(-FakeDevice.TASKS // FakeDevice.step(s))
+            assert r.heavy or r.slices == -(-FakeDevice.TASKS // FakeDevice.step(s))
+            assert r.heavy == (FakeDevice.heavy(s) and r.slices > -(-FakeDevice.TASKS // FakeDevice.step(s)))
         assert max(r.slices for r in reqs) > 1
-        # a deadline already passed: the request fails after its first slice
-        late = Request(100, "a", gp.PATTERNS["house"], deadline=0.0)
+        # requests whose tasks outlast the probe were run alone and still exact
+        assert any(r.heavy for r in reqs) and log.get("heavy_alone", 0) > 0
+        # a deadline already passed: a request that needs more than one slice
+        # fails after its first
+        name = next(n for n in ("house", "pentagon", "4clique", "rectangle", "5clique", "hourglass")
+                    if FakeDevice.step(srv.schedule_for("a", np.asarray(gp.PATTERNS[n], dtype=np.int32),
+                                                        (1, 1, 1))) < FakeDevice.TASKS)
+        late = Request(100, "a", gp.PATTERNS[name], deadline=0.0)
         srv.submit(late)
-        ok = srv.submit(Request(101, "a", gp.PATTERNS["house"]))
+        ok = srv.submit(Request(101, "a", gp.PATTERNS[name]))
         try:
             late.wait(10)
             assert False, "the deadline must fail the request"

@@ -23,7 +23,7 @@ def test_served_results_equal_alone_and_oracle():
         lat = [r.latency * 1e3 for r in reqs]
         print(f"{len(reqs)} requests in {wall:.3f} s, {len(srv.batches)} launches, p50 {percentile(lat, 50):.2f} ms, "
               f"p99 {percentile(lat, 99):.2f} ms", flush=True)
-        assert len(srv.batches) < len(reqs)
+        assert max(b[1] for b in srv.batches) > 1  # requests shared launches
         assert {r.adj.shape[0] for r in reqs} == {3, 4, 5, 6, 7}
         alone = {name: gp.DeviceGraph(h) for name, h in graphs.items()}
         for r in reqs:
