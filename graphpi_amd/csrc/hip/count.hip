@@ -60,6 +60,7 @@
 #define GP_HEAVY_CAP (1u << 20) // deferred tasks per call; the rest run in place
 #define GP_MAX_BATCH 256          // schedules per launch (8 bits of a deferred task)
 #define GP_DENSE_MAX_BYTES (64ull << 30) // pair level: dense per-wave counts only below this total
+#define GP_SCRATCH_CAP (32ull << 30)     // per-wave scratch of one call, at most (fewer waves beyond it)
 #define GP_EXT_NONE 0
 #define GP_EXT_SEARCH 1
 #define GP_EXT_PAIR 2
@@ -1715,7 +1716,17 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             dstride = parts;
             t = (e >> 24) & 0xFFFFFFFFull;
         }
-        if (GP_PLAN_EXPIRED()) continue; // (flagged)
+        if (GP_PLAN_EXPIRED()) { // (flagged)
+            if (!PARTS) {
+                // its results are discarded: the rest of its items are
+                // skipped at once, here and for every wave still to fetch
+                // (the items of the plans after it keep their order, so
+                // their counted prefixes stay exact)
+                if (lane == 0) atomicMax(a.counter, (unsigned long long)a.task_off[cur_plan + 1]);
+                cur = cend;
+            }
+            continue;
+        }
         // the task: depth 0 and 1 from the edge (v0, v1), then depth 2 on
         int32_t v0, v1 = -1;
         if (pl->root_mode) {
@@ -1984,6 +1995,18 @@ hipError_t ensure(void** p, size_t* cap, size_t bytes, hipStream_t stream) {
     *p = nullptr;
     *cap = 0;
     hipError_t e = hipMallocAsync(p, bytes, stream);
+    if (e == hipErrorOutOfMemory) {
+        // the pool keeps what calls freed (smaller blocks that a larger
+        // request cannot use): hand it back to the driver and try once more
+        (void)hipGetLastError();
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipStreamSynchronize(stream) == hipSuccess && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+            (void)hipMemPoolTrimTo(pool, 0);
+            e = hipMallocAsync(p, bytes, stream);
+        }
+    }
     if (e == hipSuccess) *cap = bytes;
     return e;
 }
@@ -2120,6 +2143,13 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
     const int64_t tmp_stride = faithful && any_k ? deg_stride : 0;
     const int64_t bslot_stride = any_batched && nbslots ? deg_stride : 0;
     const int64_t wave_stride = max_slots * slot_stride + 2 * tmp_stride + (nbslots + 1) * bslot_stride;
+    // per-wave scratch is bounded by GP_SCRATCH_CAP over all waves: a plan
+    // with many materialised sets on a graph with huge second-largest
+    // degree (7 vertices on R-MAT s24 ef16: 12 MB a wave) runs on fewer
+    // waves rather than claiming most of the device
+    if (wave_stride > 0 && (size_t)waves * (size_t)wave_stride * sizeof(int32_t) > GP_SCRATCH_CAP)
+        waves = std::max<int64_t>(std::min<int64_t>(waves, g->num_cu),
+                                  (int64_t)(GP_SCRATCH_CAP / ((size_t)wave_stride * sizeof(int32_t))));
     const size_t scratch_bytes = (size_t)std::max<int64_t>(waves * wave_stride, 64) * sizeof(int32_t);
     // Several plans share one LDS size; the larger pair-level map would cut
     // residency for all of them, so a batch gives the pair level the batched
