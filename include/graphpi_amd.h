@@ -291,7 +291,8 @@ typedef struct gp_plan_info {
     int32_t scratch_slots;   /* prefix sets materialised per wave */
     int32_t shape;           /* schedule-specialised kernel a single count takes: 0 = none (the
                                 generic kernel), 1 = House with v1 < v0 (two-hop counts per root),
-                                2 = the same House loop nest without that restriction */
+                                2 = the same House loop nest without that restriction, 3 = two
+                                triangles joined by an edge (closed form over edge supports) */
 } gp_plan_info;
 GP_API int gp_kernel_plan(const gp_sched_flat* sched, gp_plan_info* out);
 
