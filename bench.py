@@ -139,10 +139,13 @@ def batched_bench(gp, scale=12):
     dev.count_batch(scheds[:4])  # warm-up
     t0 = time.perf_counter()
     loop_ms, singles = 0.0, []
-    for sch in scheds:
+    for i, sch in enumerate(scheds):
         r = dev.count(sch)
         loop_ms += r.stats.kernel_ms
         singles.append(r.raw)
+        if r.stats.kernel_ms > 5000.0 or (i + 1) % 40 == 0:
+            log(f"batched: pattern {i + 1} of {len(scheds)}: {r.stats.kernel_ms:.0f} ms kernel "
+                f"({time.perf_counter() - t0:.1f} s so far)")
     loop_wall = time.perf_counter() - t0
     log(f"batched: one at a time {loop_wall:.2f} s ({loop_ms:.0f} ms kernel)")
     t0 = time.perf_counter()

@@ -42,6 +42,8 @@
 #define WAVE 64
 #define GP_MAX_BATCH_BLOCKS 15  // leaf blocks the batched level handles
 #define GP_LONG_LIST 64         // candidate lists at least this long are streamed by the whole wave
+#define GP_NO_HALT (~0ull)      // halt tick of a plan without a hard stop
+#define GP_HALT_STEPS 16        // long loops read the clock every this many 64-element steps
 #define GP_UNROLL 4             // independent loads in flight per lane on a long list
 // LDS hash pool per wave (dynamic shared memory, a power of two): the staged
 // leaf sets of the batched level, or the pair level's count map (keys in the
@@ -266,10 +268,16 @@ __device__ int32_t intersect_merge(const int32_t* __restrict__ Sp, int32_t ns, c
 // Otherwise, with merge_ratio > 0, lists of at least GP_MERGE_MIN elements
 // whose longer one is at most merge_ratio times the shorter are merged
 // (intersect_merge, tiles in the pool) instead of searched.
+// halt_at: a wall-clock tick past which the caller's plan is abandoned
+// (GP_NO_HALT: never).  Every GP_HALT_STEPS x 64 elements of the shorter list
+// the search checks the clock; past halt_at it raises *halt_flag and returns
+// the hits found so far (a prefix of the result, so every later access stays
+// inside the true set's bounds; the plan's sums are discarded).
 template <bool COUNT>
 __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const int32_t* __restrict__ B,
                              int32_t nb, int32_t ub, int32_t* __restrict__ outp, const int32_t* emb,
-                             int m, int32_t* pool = nullptr, int32_t pool_n = 0, int32_t merge_ratio = 0) {
+                             int m, int32_t* pool = nullptr, int32_t pool_n = 0, int32_t merge_ratio = 0,
+                             uint64_t halt_at = GP_NO_HALT, unsigned long long* halt_flag = nullptr) {
     const int lane = lane_id();
     const gptr out = GLW(outp);
     const int32_t* S = A;
@@ -309,6 +317,11 @@ __device__ int32_t intersect(const int32_t* __restrict__ A, int32_t na, const in
         if (!COUNT && hit) out[total + popc64(mk & below_me)] = x;
         total += popc64(mk);
         if (__ballot(in && !keep)) break; // the rest of S is >= ub
+        if (halt_at != GP_NO_HALT && ((base / WAVE) & (GP_HALT_STEPS - 1)) == GP_HALT_STEPS - 1 &&
+            first_lane_u64(wall_clock64()) > halt_at) {
+            if (halt_flag && lane == 0) atomicOr(halt_flag, 1ull);
+            break;
+        }
     }
     return total;
 }
@@ -532,7 +545,11 @@ __device__ __forceinline__ bool plan_expired(const KPlan* pl, uint64_t origin, u
 //   GP_EXT_MERGE:  merge-path prefix builds (gp_tuning.merge_ratio > 0).
 // A single-schedule launch needs at most one of them (a pair plan has a
 // plain leaf, a flagged plan an IEP leaf).
-template <bool FAITHFUL, bool PARTS, bool MULTI, int EXT>
+// HALT: some plan of the launch has a hard stop (KPlan::abort_ticks or
+// task_ticks); only then are the clock checks compiled in, down to the inner
+// loops of intersections and long lists, so that an abandoned plan's waves
+// leave within about a millisecond.  Without a hard stop none is read.
+template <bool FAITHFUL, bool PARTS, bool MULTI, int EXT, bool HALT>
 __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) {
     __shared__ const int32_t* s_ptr[GP_MAX_PREFIX];
     __shared__ int32_t s_len[GP_MAX_PREFIX];
@@ -572,7 +589,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     // sums are then meaningless and the host discards them; the launch's
     // other plans are unaffected.
 #define GP_PLAN_EXPIRED()                                                                                \
-    ((pl->abort_ticks > 0 || pl->task_ticks > 0) &&                                                     \
+    (HALT && (pl->abort_ticks > 0 || pl->task_ticks > 0) &&                                             \
      plan_expired(pl, first_lane_u64(s_torigin), first_lane_u64(s_ttask), &a.aborted[cur_plan]))
     // (checked per task, per loop iteration and per 64 candidates of a
     // batched or pair level)
@@ -611,6 +628,14 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         for (int i = 0; i < pl->ub_n[p]; ++i) ub = max(ub, bound(pl->ub_mask[p][i]));
         return ub;
     };
+    // the wall-clock tick past which the current plan is abandoned
+    // (GP_NO_HALT when it has no hard stop), for intersections to check
+    auto halt_at = [&]() -> uint64_t {
+        uint64_t h = GP_NO_HALT;
+        if (pl->abort_ticks > 0) h = first_lane_u64(s_torigin) + (uint64_t)pl->abort_ticks;
+        if (pl->task_ticks > 0) h = min(h, first_lane_u64(s_ttask) + (uint64_t)pl->task_ticks);
+        return h;
+    };
     auto build_prefix = [&](int p, const int32_t* nv, int32_t deg) -> int32_t {
         const int f = pl->father[p];
         const int32_t ub = ub_of(p);
@@ -625,7 +650,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             const int32_t fl = s_len[f];
             if (FAITHFUL) balg += 4ull * (uint64_t)((int64_t)fl + deg);
             len = intersect<false>(fp, fl, nv, deg, ub, out, nullptr, 0, s_hash, a.hash_slots,
-                                   EXT == GP_EXT_MERGE ? a.merge_ratio : 0);
+                                   EXT == GP_EXT_MERGE ? a.merge_ratio : 0, HALT ? halt_at() : GP_NO_HALT,
+                                   &a.aborted[cur_plan]);
             ptr = out;
             __syncthreads(); // make the new set visible to every lane
         }
@@ -654,6 +680,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         const gcptr S = GL(s_ptr[best]);
         int32_t total = 0;
         for (int32_t base = 0; base < best_len; base += WAVE) {
+            if (((base / WAVE) & (GP_HALT_STEPS - 1)) == GP_HALT_STEPS - 1 && GP_PLAN_EXPIRED()) break;
             int32_t i = base + lane;
             bool hit = i < best_len;
             int32_t x = hit ? S[i] : 0;
@@ -787,7 +814,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                         const int pi = pl->bl_src[b][i];
                         // alternate so the last intersection lands in 'out'
                         int32_t* dst = ((ns - 1 - i) & 1) ? spare : out;
-                        gn = intersect<false>(g, gn, s_ptr[pi], s_len[pi], INT_MAX, dst, nullptr, 0);
+                        gn = intersect<false>(g, gn, s_ptr[pi], s_len[pi], INT_MAX, dst, nullptr, 0, nullptr, 0, 0,
+                                              HALT ? halt_at() : GP_NO_HALT, &a.aborted[cur_plan]);
                         __syncthreads();
                         g = dst;
                     }
@@ -1016,6 +1044,9 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 lm &= lm - 1;
                 const gcptr pv = GL(a.edge) + __shfl(nvs, j);
                 const int32_t pn = __shfl(lwork, j);
+                // before a list of many steps (the shorter ones of a batch
+                // add up to little)
+                if (pn >= GP_HALT_STEPS * GP_UNROLL * WAVE && GP_PLAN_EXPIRED()) break; // (flagged)
                 if (flagged) {
                     // one pass, one search per element, every kind-1 block at
                     // once: per-lane counts, two 16-bit fields per register
@@ -1105,7 +1136,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 const int np = popc64(pm);
                 if (probe) s_plane[popc64(pm & ((1ull << lane) - 1))] = lane;
                 __syncthreads();
-                for (int b2 = 0; b2 < nb; ++b2) {
+                bool halted = false;
+                for (int b2 = 0; b2 < nb && !halted; ++b2) {
                     if (pl->bl_kind[b2] != 1) continue;
                     const int32_t gn = s_bn[b2];
                     const gcptr g = GL(s_bg[b2]);
@@ -1114,6 +1146,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                     __syncthreads();
                     const int32_t items = np * gn;
                     for (int32_t t0 = 0; t0 < items; t0 += WAVE) {
+                        if (((t0 / WAVE) & (GP_HALT_STEPS - 1)) == GP_HALT_STEPS - 1 && GP_PLAN_EXPIRED()) {
+                            halted = true; // (flagged)
+                            break;
+                        }
                         const int32_t t = t0 + lane;
                         const bool act = t < items;
                         const int32_t r = act ? t / gn : 0;
@@ -1240,6 +1276,9 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             }
             uint64_t lm = __ballot(lng);
             while (lm) {
+                // a gather may stop (flagged); a scatter or clear runs to the
+                // end so that the array is all zero again afterwards
+                if (mode == 1 && GP_PLAN_EXPIRED()) break;
                 const int j = __ffsll((long long)lm) - 1;
                 lm &= lm - 1;
                 const gcptr pv = E + __shfl(us, j);
@@ -1327,7 +1366,8 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         int32_t* keys = s_hash;
         int32_t* cnts = s_hash + pair_slots;
         const int32_t cap_elems = pair_slots / 2;
-        for (int32_t abase = 0; abase < Dn; abase += WAVE) {
+        bool halted = false; // a gather stopped at the plan's hard stop
+        for (int32_t abase = 0; abase < Dn && !halted; abase += WAVE) {
             if (GP_PLAN_EXPIRED()) break;
             const int32_t ai = abase + lane;
             bool va = ai < Dn;
@@ -1387,7 +1427,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                         }
                     }
                     __syncthreads();
-                    for (int32_t cbase = 0; cbase < Cn; cbase += WAVE) {
+                    for (int32_t cbase = 0; cbase < Cn && !halted; cbase += WAVE) {
                         const int32_t ci = cbase + lane;
                         bool vc = ci < Cn;
                         const int32_t vv = vc ? C[ci] : -1;
@@ -1404,6 +1444,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                         s_off2[lane] = vex;
                         __syncthreads();
                         for (int32_t f0 = 0; f0 < vt; f0 += WAVE) {
+                            if (((f0 / WAVE) & (GP_HALT_STEPS - 1)) == GP_HALT_STEPS - 1 && GP_PLAN_EXPIRED()) {
+                                halted = true; // (flagged)
+                                break;
+                            }
                             const int32_t f = f0 + lane;
                             const bool act = f < vt;
                             int lo = 0, hi = WAVE;
@@ -1515,6 +1559,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         }
         uint64_t lm = __ballot(lng);
         while (lm) {
+            if (mode == 1 && GP_PLAN_EXPIRED()) break; // gathers only (flagged)
             const int j = __ffsll((long long)lm) - 1;
             lm &= lm - 1;
             const gcptr pv = E + __shfl(us, j);
@@ -1563,6 +1608,7 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
         const int32_t b = bound(pl->rmask[L]);
         const int32_t* C = a.edge + l1;
         for (int32_t abase = 0; abase < Dn; abase += WAVE) {
+            if (GP_PLAN_EXPIRED()) break; // (flagged)
             const int32_t ai = abase + lane;
             int32_t av = ai < Dn ? GL(D)[ai] : -1;
             if (av == v0 || av == v1) av = -1;
@@ -1724,6 +1770,11 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
                 // their counted prefixes stay exact)
                 if (lane == 0) atomicMax(a.counter, (unsigned long long)a.task_off[cur_plan + 1]);
                 cur = cend;
+            } else if (!MULTI) {
+                // every part is the one plan's: skip them all at once (the
+                // deferred list may hold 2^20 tasks of 64 parts each)
+                if (lane == 0) atomicMax(a.counter2, (unsigned long long)n_items);
+                break;
             }
             continue;
         }
@@ -2202,24 +2253,31 @@ int prepare(const gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp
 
 // The count kernel instantiation for a launch (faithful runs have no parts
 // launch and never take the extended paths).
-const void* count_kernel_fn(bool faithful, bool parts, bool multi, int ext) {
-    if (faithful)
-        return multi ? reinterpret_cast<const void*>(&gp_count_kernel<true, false, true, GP_EXT_NONE>)
-                     : reinterpret_cast<const void*>(&gp_count_kernel<true, false, false, GP_EXT_NONE>);
+#define GP_KFN(...) reinterpret_cast<const void*>(&gp_count_kernel<__VA_ARGS__>)
+template <bool HALT>
+const void* count_kernel_pick(bool parts, bool multi, int ext) {
     // multi-plan launches have no extended instantiation
     if (parts) {
-        if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, true, GP_EXT_NONE>);
-        if (ext == GP_EXT_SEARCH) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_SEARCH>);
-        if (ext == GP_EXT_PAIR) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_PAIR>);
-        if (ext == GP_EXT_MERGE) return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_MERGE>);
-        return reinterpret_cast<const void*>(&gp_count_kernel<false, true, false, GP_EXT_NONE>);
+        if (multi) return GP_KFN(false, true, true, GP_EXT_NONE, HALT);
+        if (ext == GP_EXT_SEARCH) return GP_KFN(false, true, false, GP_EXT_SEARCH, HALT);
+        if (ext == GP_EXT_PAIR) return GP_KFN(false, true, false, GP_EXT_PAIR, HALT);
+        if (ext == GP_EXT_MERGE) return GP_KFN(false, true, false, GP_EXT_MERGE, HALT);
+        return GP_KFN(false, true, false, GP_EXT_NONE, HALT);
     }
-    if (multi) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, true, GP_EXT_NONE>);
-    if (ext == GP_EXT_SEARCH) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_SEARCH>);
-    if (ext == GP_EXT_PAIR) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_PAIR>);
-    if (ext == GP_EXT_MERGE) return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_MERGE>);
-    return reinterpret_cast<const void*>(&gp_count_kernel<false, false, false, GP_EXT_NONE>);
+    if (multi) return GP_KFN(false, false, true, GP_EXT_NONE, HALT);
+    if (ext == GP_EXT_SEARCH) return GP_KFN(false, false, false, GP_EXT_SEARCH, HALT);
+    if (ext == GP_EXT_PAIR) return GP_KFN(false, false, false, GP_EXT_PAIR, HALT);
+    if (ext == GP_EXT_MERGE) return GP_KFN(false, false, false, GP_EXT_MERGE, HALT);
+    return GP_KFN(false, false, false, GP_EXT_NONE, HALT);
 }
+
+// halt: some plan has a hard stop (the faithful loop nest never has one)
+const void* count_kernel_fn(bool faithful, bool parts, bool multi, int ext, bool halt) {
+    if (faithful)
+        return multi ? GP_KFN(true, false, true, GP_EXT_NONE, false) : GP_KFN(true, false, false, GP_EXT_NONE, false);
+    return halt ? count_kernel_pick<true>(parts, multi, ext) : count_kernel_pick<false>(parts, multi, ext);
+}
+#undef GP_KFN
 
 // Graph mode of one counting call.  The graph holds, in order: the plan
 // upload from pinned staging (the whole meta buffer), the counter reset (a
@@ -2554,6 +2612,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
                 abort_ms[p] > 0 ? std::max<int64_t>(1, (int64_t)(abort_ms[p] * g->wall_khz)) : 0;
     if (task_ms > 0)
         for (int p = 0; p < nplans; ++p) ln.kps[p].task_ticks = std::max<int64_t>(1, (int64_t)(task_ms * g->wall_khz));
+    bool halt = false; // the kernel instantiation with the hard-stop checks
+    for (int p = 0; p < nplans; ++p) halt = halt || ln.kps[p].abort_ticks > 0 || ln.kps[p].task_ticks > 0;
     const bool faithful = ln.faithful, any_split = ln.any_split;
     const std::vector<KPlan>& kps = ln.kps;
     const std::vector<uint64_t>&task_off = ln.task_off, &task_first = ln.task_first, &task_cnt = ln.task_cnt;
@@ -2662,8 +2722,8 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
     bool launched = false;
     if (tuning && tuning->use_graph && n_tasks > 0) {
         // the same device work as below, as one replayable graph
-        const void* kfun = count_kernel_fn(faithful, false, multi, ext);
-        const void* pfun = count_kernel_fn(false, true, multi, ext);
+        const void* kfun = count_kernel_fn(faithful, false, multi, ext, halt);
+        const void* pfun = count_kernel_fn(false, true, multi, ext, halt);
         KArgs a2 = a;
         a2.clocks = c->d_clocks + 2 * waves;
         const size_t n_clk = (parts_launch ? 4 : 2) * (size_t)waves;
@@ -2680,13 +2740,15 @@ int run_plans(gp_graph* g, const gp_sched_flat* scheds, int nplans, const gp_tun
         if (e == hipSuccess && n_tasks > 0) {
             const dim3 grid((unsigned)waves), block(WAVE);
             void* kargs[] = {&a};
-            e = hipLaunchKernel(count_kernel_fn(faithful, false, multi, ext), grid, block, kargs, lds_bytes, c->stream);
+            e = hipLaunchKernel(count_kernel_fn(faithful, false, multi, ext, halt), grid, block, kargs, lds_bytes,
+                                c->stream);
             if (e == hipSuccess && parts_launch) {
                 // the deferred tasks, in parts; the kernel reads how many there are
                 KArgs a2 = a;
                 a2.clocks = c->d_clocks + 2 * waves;
                 void* pargs[] = {&a2};
-                e = hipLaunchKernel(count_kernel_fn(false, true, multi, ext), grid, block, pargs, lds_bytes, c->stream);
+                e = hipLaunchKernel(count_kernel_fn(false, true, multi, ext, halt), grid, block, pargs, lds_bytes,
+                                    c->stream);
             }
         }
         if (e == hipSuccess) e = hipEventRecord(c->e1, c->stream);
@@ -3065,7 +3127,12 @@ int count_ranged(gp_graph* g, const gp_sched_flat* sched, const gp_tuning* tunin
             rc = run_plans(g, sched, 1, &sub, &r, nullptr, &st, nullptr, nullptr, t.slice_ms < 0 ? 0 : slice, &did,
                            hard ? &abort_ms : nullptr, &ab);
             if (rc != GP_OK) return rc;
-            if (ab) { // abandoned: the launches before it stand
+            if (ab) { // abandoned: the launches before it stand; its time counts
+                acc.finish_p50_ms = acc.kernel_ms + st.finish_p50_ms;
+                acc.finish_p90_ms = acc.kernel_ms + st.finish_p90_ms;
+                acc.finish_max_ms = acc.kernel_ms + st.finish_max_ms;
+                acc.kernel_ms += st.kernel_ms;
+                ++acc.launches;
                 timed_out = true;
                 break;
             }
@@ -3228,13 +3295,13 @@ extern "C" int gp_count_noalloc(const gp_graph* gc, const gp_sched_flat* sched, 
     }
     if (e == hipSuccess && ln.n_tasks > 0) {
         void* kargs[] = {&a};
-        e = hipLaunchKernel(count_kernel_fn(ln.faithful, false, false, ext), dim3((unsigned)waves), dim3(WAVE), kargs,
-                            ln.lds_bytes, 0);
+        e = hipLaunchKernel(count_kernel_fn(ln.faithful, false, false, ext, false), dim3((unsigned)waves), dim3(WAVE),
+                            kargs, ln.lds_bytes, 0);
         if (e == hipSuccess && parts) {
             KArgs a2 = a;
             a2.clocks = clk + 2 * waves;
             void* pargs[] = {&a2};
-            e = hipLaunchKernel(count_kernel_fn(false, true, false, ext), dim3((unsigned)waves), dim3(WAVE), pargs,
+            e = hipLaunchKernel(count_kernel_fn(false, true, false, ext, false), dim3((unsigned)waves), dim3(WAVE), pargs,
                                 ln.lds_bytes, 0);
         }
     }

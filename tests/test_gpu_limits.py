@@ -289,30 +289,35 @@ def test_progress_batches_resume_exactly():
 
 def test_hard_limit_abandons_a_long_launch():
     """gp_tuning.abort_s: a launch still running at the limit abandons its
-    work wherever its waves are, and gp_count returns GP_ETIME soon after,
-    with the result of the launches that completed before it (a prefix of
-    the tasks)."""
+    work wherever its waves are (inside intersections and long lists too),
+    and gp_count returns GP_ETIME right after, with the result of the
+    launches that completed before it (a prefix of the tasks)."""
     import time
     host = gp.HostGraph.rmat(16, 16, 0.57, 0.19, 0.19, 4)
     dev = gp.DeviceGraph(host)
     tri = dev.triangle_counting()
-    # a 6-vertex motif whose full count here takes minutes, as one launch
-    # (and single tasks that run longer than the limit); the 5-clique (about
-    # 240 ms here) in launches of about 5 ms
+    # a 6-vertex motif whose full count here takes minutes, as one launch,
+    # with single tasks that run longer than the limit: nothing completes
     heavy, _ = gp.build_schedule(gp.motifs(6)[77], 1, 1, True, host.v_cnt, host.e_cnt, tri)
-    five, _ = gp.build_schedule(gp.PATTERNS["5clique"], 1, 1, True, host.v_cnt, host.e_cnt, tri)
-    for s, slice_ms, limit in ((heavy, -1, 0.3), (five, 5, 0.1)):
+    # the 4-clique on R-MAT scale 22 (about 330 ms, no task longer than about
+    # a millisecond) in launches of about 5 ms: those before the limit stand
+    big = gp.HostGraph.rmat(22, 8, 0.57, 0.19, 0.19, 2)
+    bdev = gp.DeviceGraph(big)
+    btri = bdev.triangle_counting()
+    four, _ = gp.build_schedule(gp.PATTERNS["4clique"], 1, 1, True, big.v_cnt, big.e_cnt, btri)
+    for d, s, slice_ms, limit in ((dev, heavy, -1, 0.3), (bdev, four, 5, 0.15)):
         t = gp.Tuning()
         t.slice_ms = slice_ms
         t.abort_s = limit
         t0 = time.perf_counter()
         with pytest.raises(gp.GPTimeout) as e:
-            dev.count(s, t)
+            d.count(s, t)
         wall = time.perf_counter() - t0
         part = e.value.partial
-        print(f"slice_ms {slice_ms}: stopped after {wall:.3f} s, {part.stats.launches} launches completed, "
-              f"{part.stats.tasks} tasks", flush=True)
-        assert wall < 2.0
+        print(f"slice_ms {slice_ms}: stopped after {wall:.3f} s, {part.stats.launches} launches "
+              f"({part.stats.kernel_ms:.1f} ms), {part.stats.tasks} of {part.stats.tasks_total} tasks", flush=True)
+        # the waves leave within milliseconds of the limit
+        assert wall < limit + 0.25
         if slice_ms < 0:
             assert part.stats.tasks == 0 and part.raw == 0
         else:
@@ -320,7 +325,7 @@ def test_hard_limit_abandons_a_long_launch():
             r = gp.Tuning()
             r.task_end = int(part.stats.tasks)
             r.slice_ms = -1
-            assert dev.count(s, r).raw == part.raw
+            assert d.count(s, r).raw == part.raw
     # a limit that is not reached changes nothing
     t = gp.Tuning()
     t.abort_s = 600.0
