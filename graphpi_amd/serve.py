@@ -3,7 +3,7 @@
 A CountServer keeps several graphs resident in HBM and answers counting
 requests (a pattern, its planner settings and a graph name).  Worker slots
 take requests continuously: a slot that frees up immediately gathers the
-requests waiting for one graph (oldest first), up to a batch size and within
+requests waiting for one graph (smallest patterns first), up to a batch size and within
 the device-memory budget (each batch's footprint comes from
 gp_count_footprint), and counts them together with
 gp_count_batch_progress.  Slots run on their own threads and HIP streams, so
@@ -103,7 +103,8 @@ class CountServer:
     2 x quantum).
 
     Scheduling.  New requests of one graph are counted together in a first
-    slice with the per-task probe: most small counts finish there.  A request
+    slice with the per-task probe, the smallest patterns first: most small
+    counts finish there.  A request
     still unfinished (its slice ran out, or one of its tasks ran past the
     probe, which abandons it for that call without touching the others)
     joins the continuing queue with its progress.  That queue runs shortest
@@ -140,11 +141,15 @@ class CountServer:
         self._tuning = tuning
         # continuing requests: the time slice, the deadline as the hard stop
         self._slice_tuning = self._tuning_for_slice()
-        # new requests: also the per-fetch probe (the parts of deferred heavy
-        # tasks run in a second launch, fetch by fetch, under the same probe)
+        # new requests: also the per-fetch probe
         self._new_tuning = self._tuning_for_slice()
         if self._probe:
             self._new_tuning.task_abort_s = self._probe
+            # and no deferred heavy tasks: their parts run after the slice's
+            # deadline, each under the probe, so a request of many heavy
+            # tasks would hold the device until its abort time; in place, a
+            # task that runs past the probe marks its request heavy at once
+            self._new_tuning.split_min = -1
         self._max_batch = max_batch
         if mem_budget is None:
             # most of what is free once the graphs are resident (the rest is
@@ -254,21 +259,31 @@ class CountServer:
         r.t_done = now
         r._event.set()
 
-    def _take_new(self):
-        """Called with the lock held: new requests of the oldest new request's
-        graph and launch class, up to max_batch.  (A batch call runs one
+    def _take_new(self, now):
+        """Called with the lock held: new requests of one graph and launch
+        class, up to max_batch: those of the pending request with the least
+        static cost (pattern vertices, then the graph's tasks), shortest job
+        first, since under load the small requests are the ones that can
+        still meet their deadlines.  Pending requests whose deadline has
+        already passed fail here without a slice.  (A batch call runs one
         launch per class of LDS size in turn; with one class per call its
         whole time slice goes to the one launch, and no request waits behind
         another class's launch for a slice that never reaches it.)"""
-        graph, cls = self._pending[0].graph, self._pending[0].cls
-        batch, rest = [], collections.deque()
-        while self._pending:
-            r = self._pending.popleft()
-            if r.graph == graph and r.cls == cls and len(batch) < self._max_batch:
-                batch.append(r)
+        live = collections.deque()
+        for r in self._pending:
+            if r.deadline is not None and now - r.t_arrival >= r.deadline:
+                self._fail_timeout(r, now)
             else:
-                rest.append(r)
-        self._pending = rest
+                live.append(r)
+        self._pending = live
+        if not live:
+            return []
+        cost = lambda r: (r.adj.shape[0], r.total, r.t_arrival)
+        head = min(live, key=cost)
+        graph, cls = head.graph, head.cls
+        batch = sorted((r for r in live if r.graph == graph and r.cls == cls), key=cost)[:self._max_batch]
+        taken = set(id(r) for r in batch)
+        self._pending = collections.deque(r for r in live if id(r) not in taken)
         return batch
 
     def _take_long(self, now):
@@ -297,11 +312,11 @@ class CountServer:
         new_first = slot == 0 or not self._long
         batch, new = [], False
         if new_first and self._pending:
-            batch, new = self._take_new(), True
+            batch, new = self._take_new(now), True
         if not batch and self._long:
-            batch = self._take_long(now)
+            batch, new = self._take_long(now), False
         if not batch and self._pending:
-            batch, new = self._take_new(), True
+            batch, new = self._take_new(now), True
         if not batch:
             return None
         graph = batch[0].graph

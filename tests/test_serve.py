@@ -183,8 +183,8 @@ def test_long_counts_resume_and_deadlines_fail_alone():
         assert max(r.slices for r in reqs) > 1
         # requests whose tasks outlast the probe were run alone and still exact
         assert any(r.heavy for r in reqs) and log.get("heavy_alone", 0) > 0
-        # a deadline already passed: a request that needs more than one slice
-        # fails after its first
+        # a deadline already passed when the request is taken: it fails
+        # without using the device, and the others are unaffected
         name = next(n for n in ("house", "pentagon", "4clique", "rectangle", "5clique", "hourglass")
                     if FakeDevice.step(srv.schedule_for("a", np.asarray(gp.PATTERNS[n], dtype=np.int32),
                                                         (1, 1, 1))) < FakeDevice.TASKS)
@@ -195,5 +195,14 @@ def test_long_counts_resume_and_deadlines_fail_alone():
             late.wait(10)
             assert False, "the deadline must fail the request"
         except RequestTimeout as e:
-            assert late.timed_out and 0 < e.done < e.total
+            assert late.timed_out and e.done == 0 and late.slices == 0
         assert ok.wait(10) == FakeDevice.value(ok.sched)
+        # a deadline that passes during a count that needs more than one
+        # slice fails it after a slice, with its progress so far
+        slow = Request(102, "a", gp.PATTERNS[name], deadline=0.001)
+        srv.submit(slow)
+        try:
+            slow.wait(10)
+            assert False, "the deadline must fail the request"
+        except RequestTimeout as e:
+            assert slow.timed_out and e.done < e.total
