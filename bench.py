@@ -193,8 +193,10 @@ def served_bench(gp, scales=(16, 20, 22, 24), rates=(50, 200, 800), n_req=60, se
     p99 are over answered requests; timed-out requests are reported by class
     (pattern size, graph), never dropped.  Throughput counts answered
     requests.  Every answered result equals the count made alone."""
+    import gc
     import numpy as np
     from graphpi_amd.serve import CountServer, Request, percentile, poisson_arrivals, random_pattern, run_load
+    gc.collect()  # device objects left from earlier legs are freed now, not during the load
     t0 = time.perf_counter()
     graphs = {}
     for i, sc in enumerate(scales):

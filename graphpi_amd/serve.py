@@ -3,8 +3,9 @@
 A CountServer keeps several graphs resident in HBM and answers counting
 requests (a pattern, its planner settings and a graph name).  Worker slots
 take requests continuously: a slot that frees up immediately gathers the
-requests waiting for one graph (smallest patterns first), up to a batch size and within
-the device-memory budget (each batch's footprint comes from
+requests waiting for one graph (those with deadlines first, the smallest
+patterns first among them, then the others oldest first), up to a batch size
+and within the device-memory budget (each batch's footprint comes from
 gp_count_footprint), and counts them together with
 gp_count_batch_progress.  Slots run on their own threads and HIP streams, so
 a batch that arrives while others are on the GPU starts as soon as a slot and
@@ -103,8 +104,8 @@ class CountServer:
     2 x quantum).
 
     Scheduling.  New requests of one graph are counted together in a first
-    slice with the per-task probe, the smallest patterns first: most small
-    counts finish there.  A request
+    slice with the per-task probe (requests with deadlines first, the
+    smallest patterns first among them): most small counts finish there.  A request
     still unfinished (its slice ran out, or one of its tasks ran past the
     probe, which abandons it for that call without touching the others)
     joins the continuing queue with its progress.  That queue runs shortest
@@ -194,11 +195,19 @@ class CountServer:
         return req
 
     def close(self):
+        """Finish the queued requests, stop the slots and free the device
+        graphs (and their call buffers) now, rather than whenever the
+        garbage collector reaches them: a free synchronises the device and
+        would stall whatever runs on it then."""
         with self._cv:
             self._closed = True
             self._cv.notify_all()
         for w in self._workers:
             w.join()
+        self._workers = []
+        for d in self._dev.values():
+            if hasattr(d, "free"):
+                d.free()
 
     def __enter__(self):
         return self
@@ -261,10 +270,12 @@ class CountServer:
 
     def _take_new(self, now):
         """Called with the lock held: new requests of one graph and launch
-        class, up to max_batch: those of the pending request with the least
-        static cost (pattern vertices, then the graph's tasks), shortest job
-        first, since under load the small requests are the ones that can
-        still meet their deadlines.  Pending requests whose deadline has
+        class, up to max_batch.  Requests with a deadline come first, the
+        one with the least static cost (pattern vertices, then the graph's
+        tasks) picking the group: shortest job first, since under load the
+        small requests are the ones that can still meet their deadlines.
+        Requests without one follow in arrival order (shortest first would
+        only hold the large ones back).  Pending requests whose deadline has
         already passed fail here without a slice.  (A batch call runs one
         launch per class of LDS size in turn; with one class per call its
         whole time slice goes to the one launch, and no request waits behind
@@ -278,7 +289,8 @@ class CountServer:
         self._pending = live
         if not live:
             return []
-        cost = lambda r: (r.adj.shape[0], r.total, r.t_arrival)
+        cost = lambda r: ((0, r.adj.shape[0], r.total, r.t_arrival) if r.deadline is not None
+                          else (1, 0, 0, r.t_arrival))
         head = min(live, key=cost)
         graph, cls = head.graph, head.cls
         batch = sorted((r for r in live if r.graph == graph and r.cls == cls), key=cost)[:self._max_batch]
