@@ -522,7 +522,9 @@ def main():
         if n in kernel_ms:
             continue
         log(f"  counting {n} once ...")
-        once[n] = count_pattern(n)
+        # longer launches than the timed patterns': each launch ends with a
+        # tail (the fetches in flight at its deadline), so fewer is cheaper
+        once[n] = count_pattern(n, target_ms=15000.0)
         log(f"  {n}: {once[n]['count']} in {once[n]['kernel_ms']:.1f} ms kernel time ({once[n]['launches']} launches)"
             + (f"; without wrap-around {once[n]['exact']}" if once[n].get("exact") is not None else ""))
         kernel_ms[n] = once[n]["kernel_ms"]
@@ -544,7 +546,9 @@ def main():
         for n in cfg["patterns"] + [x for x in cfg["once"] if x not in cfg["patterns"]]:
             rs, valid = gp.build_schedule(gp.PATTERNS[n], 1, 1, True, rhost.v_cnt, rhost.e_cnt, rtri)
             assert valid
-            runs = [rdev.count(rs) for _ in range(3 if n in cfg["patterns"] else 1)]
+            rt = gp.Tuning()
+            rt.slice_ms = 2000 if n in cfg["patterns"] else 15000
+            runs = [rdev.count(rs, rt) for _ in range(3 if n in cfg["patterns"] else 1)]
             assert all(r.count == counts[n] for r in runs), (n, [r.count for r in runs], counts[n])
             best = min(r.stats.kernel_ms for r in runs)
             relabelled["per_pattern"][n] = {"count": runs[0].count, "kernel_ms": round(best, 3),

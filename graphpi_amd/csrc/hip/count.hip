@@ -1653,11 +1653,15 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
     // (the fetch's end, size and start time live in LDS: they are touched
     // once per fetch, and registers held across the whole loop spill)
     // With a.deadline_ticks > 0 a wave stops fetching once that many ticks
-    // have passed since it started (after its first fetch, so every launch
-    // makes progress).  Items are handed out in order from one counter and a
-    // fetched item is always finished, so the launch then has counted
-    // exactly the items below the counter's final value: a prefix, which the
-    // host resumes from.
+    // have passed since the launch's first wave started (s_torigin).  The
+    // grid can hold more waves than fit on the device at once (LDS bounds
+    // residency), and a wave that starts only when another exits then finds
+    // the deadline passed: it fetches nothing once any wave has fetched (so
+    // every launch makes progress), instead of running a deadline of its own
+    // after the first ones, which made launches last two deadlines.  Items
+    // are handed out in order from one counter and a fetched item is always
+    // finished, so the launch then has counted exactly the items below the
+    // counter's final value: a prefix, which the host resumes from.
     __shared__ uint64_t s_fend, s_tfetch, s_tstart;
     __shared__ int32_t s_chunk;
     if (lane == 0) {
@@ -1686,8 +1690,10 @@ __global__ void __launch_bounds__(WAVE, MULTI ? 5 : 6) gp_count_kernel(KArgs a) 
             while (cur >= cend) {
                 if (cur >= first_lane_u64(s_fend)) {
                     if (EXT == GP_EXT_PAIR) pv_clear(); // the dense array is zero between fetches
-                    if (a.deadline_ticks > 0 && first_lane_u64(s_fend) > 0 &&
-                        first_lane_u64(wall_clock64()) - first_lane_u64(s_tstart) > (uint64_t)a.deadline_ticks) {
+                    if (a.deadline_ticks > 0 &&
+                        first_lane_u64(wall_clock64()) - first_lane_u64(s_torigin) > (uint64_t)a.deadline_ticks &&
+                        (first_lane_u64(s_fend) > 0 ||
+                         first_lane_u64(__hip_atomic_load(a.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) > 0)) {
                         done = true;
                         break;
                     }
