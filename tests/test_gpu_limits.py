@@ -331,3 +331,34 @@ def test_hard_limit_abandons_a_long_launch():
     t.abort_s = 600.0
     s4, _ = gp.build_schedule(gp.PATTERNS["4clique"], 1, 1, True, host.v_cnt, host.e_cnt, tri)
     assert dev.count(s4, t).raw == dev.count(s4).raw
+
+
+def test_unreached_hard_stop_changes_nothing_on_every_path():
+    """A hard stop that is never reached runs the kernel instantiation with
+    the clock checks (one per extended path: plain, flagged shared search,
+    pair level grouped by v1, merge-path builds) and must give the same raw
+    sums as the instantiation without them."""
+    host = gp.HostGraph.rmat(16, 16, 0.57, 0.19, 0.19, 4)
+    dev = gp.DeviceGraph(host)
+    tri = dev.triangle_counting()
+
+    def sched(name):
+        return gp.build_schedule(gp.PATTERNS[name], 1, 1, True, host.v_cnt, host.e_cnt, tri)[0]
+
+    house, pent, four = sched("house"), sched("pentagon"), sched("4clique")
+    assert gp.kernel_plan(house)["shared_search"] and gp.kernel_plan(pent)["pair_grouped_v1"]
+    cases = [("4clique", four, {}), ("house generic", house, {"no_special": 1}),
+             ("pentagon", pent, {}), ("4clique merge path", four, {"merge_ratio": 8})]
+    for name, s, fields in cases:
+        base = gp.Tuning()
+        for k, v in fields.items():
+            setattr(base, k, v)
+        want = dev.count(s, base).raw
+        hard = gp.Tuning()
+        for k, v in fields.items():
+            setattr(hard, k, v)
+        hard.abort_s = 600.0
+        hard.task_abort_s = 600.0
+        got = dev.count(s, hard).raw
+        print(f"{name}: {got} (no hard stop {want})", flush=True)
+        assert got == want, name
