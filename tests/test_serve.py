@@ -206,3 +206,36 @@ def test_long_counts_resume_and_deadlines_fail_alone():
             assert False, "the deadline must fail the request"
         except RequestTimeout as e:
             assert slow.timed_out and e.done < e.total
+
+
+def test_new_requests_order_deadlines_smallest_first():
+    """The new-request queue: requests already past their deadline fail
+    without a slice; those with a deadline are taken smallest pattern first;
+    those without one follow in arrival order."""
+    from graphpi_amd.serve import RequestTimeout
+    log = {"in_flight": 0, "peak": 0, "sizes": []}
+    graphs = {"a": gp.HostGraph.complete(8)}
+    srv = CountServer(graphs, slots=0, device_factory=lambda h: FakeDevice(h, log))  # no slots: the test drives
+    t = time.perf_counter()
+    big = Request(0, "a", gp.PATTERNS["house"], t_arrival=t - 3.0, deadline=100.0)
+    small = Request(1, "a", gp.PATTERNS["triangle"], t_arrival=t - 2.0, deadline=100.0)
+    free_old = Request(2, "a", gp.PATTERNS["rectangle"], t_arrival=t - 5.0)
+    free_new = Request(3, "a", gp.PATTERNS["4clique"], t_arrival=t - 1.0)
+    late = Request(4, "a", gp.PATTERNS["triangle"], t_arrival=t - 2.0, deadline=1.0)
+    for r in (big, small, free_old, free_new, late):
+        srv.submit(r)
+    order = []
+    with srv._cv:
+        while srv._pending:
+            batch = srv._take_new(time.perf_counter())
+            order.append([r.rid for r in batch])
+    assert late.timed_out and late.slices == 0 and late.done == 0
+    try:
+        late.wait(0)
+        assert False
+    except RequestTimeout:
+        pass
+    flat = [rid for b in order for rid in b]
+    assert sorted(flat) == [0, 1, 2, 3]
+    assert flat.index(1) < flat.index(0) < flat.index(2) < flat.index(3)
+    srv.close()
