@@ -124,11 +124,13 @@ typedef struct gp_tuning {
     int32_t bitmap_ratio;   /* batched level: a set goes into an LDS bitmap when the bitmap's
                                words are at most this many times its sorted table's entries;
                                0 = default (16), < 0 = never */
-    int32_t slice_ms;       /* gp_count runs a long count as consecutive launches over task
-                               ranges, each sized from the previous one's rate for about this
-                               many milliseconds of kernel time (a first launch takes 1/256 of
-                               the tasks when there are more than 2^20); 0 = default (2000),
-                               < 0 = one launch */
+    int32_t slice_ms;       /* gp_count runs a long count as consecutive launches of about this
+                               many milliseconds: on the generic kernel each launch stops
+                               fetching tasks at that deadline (counted from its first wave's
+                               start) and the next resumes at the first task not counted; the
+                               House kernel takes root ranges sized from the previous range's
+                               rate.  A launch overruns by the fetches in flight and its
+                               deferred parts.  0 = default (2000), < 0 = one launch */
     int32_t no_special;     /* 1 = never use a schedule-specialised kernel (House under the
                                GraphPi schedule is counted per root with two-hop counts and
                                edge supports by default); counts are identical either way */
@@ -147,9 +149,12 @@ typedef struct gp_tuning {
     double abort_s;         /* gp_count, gp_count_batch_progress: when > 0, a launch still running
                                this long after the call started abandons its work (wherever its
                                waves are), and the call returns GP_ETIME with the result of the
-                               launches that completed before it (a prefix of the tasks).  For
-                               callers that must answer by a deadline even when one task would run
-                               past it (a serving loop).  Such a gp_count takes the generic kernel */
+                               launches that completed before it (a prefix of the tasks); the clock
+                               is read down to the intersection loops, so the call returns within
+                               about 15 ms of the limit.  For callers that must answer by a
+                               deadline even when one task would run past it (a serving loop).
+                               Such a gp_count takes the generic kernel, in the instantiation with
+                               these checks (counts without a hard stop compile them out) */
     double task_abort_s;    /* gp_count_batch_progress: when > 0, a schedule one of whose work
                                fetches (a few tasks, sized for about a millisecond from the wave's
                                last fetch) has run this long is abandoned as at its abort time
