@@ -1,7 +1,8 @@
 """Write the golden fixtures in this folder from the reference build.
 
 Runs oracle/_ref/ref_driver (the unmodified reference sources compiled by
-oracle/Makefile) in the build container only; the GPU box never runs it.
+oracle/Makefile) where the reference sources are present; the committed
+fixtures are what every other machine reads.
 
     python tests/golden/make_goldens.py [--quick]
 
